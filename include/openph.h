@@ -1,0 +1,143 @@
+/*
+ * openph - MI355X (gfx950) Z/2 boundary-matrix reduction, C ABI.
+ *
+ * Conventions shared with the reference (src/cuda/openph.cu:7-27):
+ *   - the boundary matrix is an m x m Z/2 matrix given as 1-based COO
+ *     triplets (rows1[e], cols1[e]), e < nnz, column-major with rows
+ *     strictly ascending inside a column and columns non-decreasing
+ *     (src/cuda/openph.cu:44, src/cuda/kernels.cu:27,42);
+ *   - every entry satisfies 1 <= rows1[e] < cols1[e] <= m (a filtration
+ *     order: the reference assumes it through its clearing rules and never
+ *     checks it; here it is validated);
+ *   - low is returned 1-based with 0 meaning "the reduced column is zero"
+ *     (src/cuda/openph.cu:195-198), ess[j] = 1 iff cell j is essential
+ *     (src/cuda/openph.cu:199).
+ *
+ * Results are unique for a given matrix: every algorithm string returns the
+ * same low and ess arrays.
+ */
+#ifndef OPENPH_H
+#define OPENPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes returned by the v2 entry points (0 = success). */
+#define OPENPH_OK 0
+#define OPENPH_EINVAL -1      /* bad argument or malformed matrix            */
+#define OPENPH_EUNSORTED -2   /* COO not column-major / rows not ascending   */
+#define OPENPH_EALGO -3       /* unknown algorithm string                    */
+#define OPENPH_ENOMEM -4      /* device memory (or the pool budget) exhausted */
+#define OPENPH_EDEVICE -5     /* HIP runtime error                           */
+#define OPENPH_EITER -6       /* iteration cap reached                       */
+#define OPENPH_EDIMS -7       /* more than OPENPH_MAX_DIMS pseudo-dimensions */
+#define OPENPH_EBUSY -8       /* served queue full / admission refused       */
+
+#define OPENPH_MAX_DIMS 16
+
+/*
+ * Drop-in replacement for the reference's host entry point
+ * (src/cuda/openph.cu:29-35, called by the mex gateway
+ * src/matlab/ph/ph.cu:84-87). Same parameter order and types.
+ *
+ *   algstr      "pms" | "standard" | "twist" | "ph_row" |
+ *               "standard_parallel" | "twist_parallel" | "ph_row_parallel"
+ *               (src/cuda/algorithms/algorithm_factory.cu:10-148)
+ *   col_width   accepted and ignored: columns live in an adaptive device
+ *               pool, there is no fixed p = col_width * max_nnz.
+ *   h_low_true  1-based true low array used only by the trackers (may be
+ *               all zeros, as numerics/benchmark_datasets.m:53 passes).
+ *   outputs     h_low[m], h_ess[m]; err_linf, err_lone, err_redu, err_ess,
+ *               time_track: m+1 floats each (index = iteration);
+ *               num_iters[0].
+ *
+ * The reference has no error channel. Here errors never exit the process:
+ * on failure num_iters[0] is set to the negative status code and h_low is
+ * left as the caller initialised it (the mex gateway fills -1,
+ * src/matlab/ph/ph.cu:72).
+ */
+void openph(char *algstr, int *h_rows_in, int *h_cols_in, int m, int col_width,
+            int *h_low_true, int nnz, int *h_low, int *h_ess, float *err_linf,
+            float *err_lone, float *err_redu, float *err_ess, float *time_track,
+            int *num_iters, int gpu_number);
+
+/* Options for the v2 / device entry points. Zero-initialise, then set. */
+typedef struct openph_options {
+    int device;              /* HIP device ordinal                           */
+    int max_iters;           /* pms iteration cap (0 = 1,000,000)            */
+    int64_t pool_elems;      /* initial column-pool size in int32 entries
+                                (0 = automatic)                              */
+    int64_t pool_max_elems;  /* largest pool the reduction may grow to
+                                (0 = 2^32 - 1 entries, 16 GiB)               */
+    int lanes_per_column;    /* Phase II lanes per column: 0 = autotuned,
+                                else 1, 2, 4, 8, 16, 32 or 64                */
+    int track;               /* 1 = fill the per-iteration trackers          */
+} openph_options;
+
+/* Statistics of one reduction. */
+typedef struct openph_stats {
+    int num_iters;           /* the reference's num_iters (iterations + 1)   */
+    int compactions;         /* pool compactions performed                   */
+    int64_t additions;       /* column additions performed                   */
+    int64_t merge_elems;     /* sum over additions of |a| + |b| + |a xor b|  */
+    int64_t pool_peak;       /* largest pool fill seen, in int32 entries     */
+    float device_ms;         /* device time of the reduction loop            */
+    float phase2_ms;         /* summed device time of the Phase II launches  */
+    int phase2_launches;     /* number of Phase II launches                  */
+    int m;                   /* columns reduced                              */
+} openph_stats;
+
+/*
+ * Status-returning variant of openph() on host buffers. low1 and ess are
+ * m-element outputs (1-based low, 0/1 ess); stats may be NULL.
+ * trackers (may be NULL) receives 5 x (m+1) floats laid out as
+ * err_linf | err_lone | err_redu | err_ess | time_track and is only written
+ * when opt->track is set.
+ */
+int openph_v2(const char *algstr, const int *rows1, const int *cols1, int m, int nnz,
+              const int *low_true1, const openph_options *opt, int *low1, int *ess,
+              float *trackers, openph_stats *stats);
+
+/*
+ * Device-resident entry point: all array arguments are device pointers on
+ * opt->device, the work is queued on `stream` (a hipStream_t, NULL = the
+ * default stream) and the call returns when the results are in low1/ess.
+ * Library workspace is allocated on first use and kept for later calls.
+ */
+int openph_pms_device(const int *rows1, const int *cols1, int m, int nnz,
+                      int *low1, int *ess, void *stream, const openph_options *opt,
+                      openph_stats *stats);
+
+/*
+ * Batched reduction of `count` independent matrices in one pass of the
+ * device engine (block-diagonal offsets). The inputs are concatenated:
+ * matrix k has m_k = ms[k] columns and nnz_k = nnzs[k] entries and its COO
+ * (1-based, local to the matrix) occupies rows1/cols1[nnz_off_k ..], with
+ * nnz_off_k the prefix sum of nnzs. Outputs are concatenated the same way
+ * over the m_k (low local and 1-based). Host pointers.
+ */
+int openph_pms_batch(int count, const int *ms, const int *nnzs, const int *rows1,
+                     const int *cols1, const openph_options *opt, int *low1, int *ess,
+                     openph_stats *stats);
+
+/* Same as openph_pms_batch with device pointers for rows1/cols1/low1/ess
+ * (ms and nnzs stay on the host). */
+int openph_pms_batch_device(int count, const int *ms, const int *nnzs, const int *rows1,
+                            const int *cols1, int *low1, int *ess, void *stream,
+                            const openph_options *opt, openph_stats *stats);
+
+/* Human-readable message for a status code. */
+const char *openph_strerror(int status);
+
+/* Library version string. */
+const char *openph_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENPH_H */

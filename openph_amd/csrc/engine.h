@@ -1,0 +1,76 @@
+// Internal interface between the host orchestration (openph.cpp) and the
+// gfx950 kernels (kernels.hip). Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace oph {
+
+constexpr int kMaxSlots = 16;      // distinct pseudo-dimensions supported
+constexpr int kBlock = 256;        // threads per block for flat passes
+constexpr int kItems = 4;          // columns per thread in the Phase I scan
+constexpr int kTile = kBlock * kItems;
+constexpr int kIntMax = 0x7fffffff;
+
+// Device-side scalars shared by all kernels of one reduction. Lives in
+// device memory; the host reads it back with one small copy.
+struct Ctrl {
+    unsigned long long pool_top;   // next free pool entry
+    unsigned long long pool_cap;   // pool capacity (entries)
+    unsigned long long additions;  // column additions performed
+    unsigned long long merge_elems;// sum |a|+|b|+|a xor b|
+    int err;                       // validation error bits
+    int any_dup;                   // a non-negative low occurs twice
+    int overflow;                  // Phase II ran out of pool
+    int nslots;                    // distinct pseudo-dimensions
+    int maxlen;                    // longest input column
+    int marks;                     // columns marked by Phase I (cumulative)
+    int pad[2];
+};
+
+// Validation error bits (Ctrl::err).
+enum : int {
+    kErrRange = 1,      // index outside 1..m or row >= col
+    kErrOrder = 2,      // columns decreasing
+    kErrRows = 4,       // rows not strictly ascending inside a column
+    kErrSlots = 8,      // more than kMaxSlots pseudo-dimensions
+    kErrState = 16,     // internal consistency check failed
+};
+
+// All device buffers of one reduction. Sized for m columns and a pool.
+struct Dev {
+    int m = 0;
+    int nnz = 0;
+    int ntiles = 0;
+    // per column
+    int *low = nullptr, *arglow = nullptr, *classes = nullptr, *ess = nullptr;
+    int *left = nullptr, *beta = nullptr, *first = nullptr, *clear = nullptr;
+    int *newpiv = nullptr, *slot = nullptr, *len = nullptr;
+    unsigned *off = nullptr, *off2 = nullptr;
+    unsigned long long *scan_tmp = nullptr;  // compaction scan partials
+    int *slotmap = nullptr;                  // pseudo-dim + 1 -> slot
+    int *tile_max = nullptr, *tile_pre = nullptr;
+    int *pool = nullptr, *pool2 = nullptr;   // column pool (int32 rows)
+    Ctrl *ctrl = nullptr;
+};
+
+// ---- launchers (kernels.hip) ----
+void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t s);
+void launch_slots(const Dev &d, hipStream_t s);
+void launch_phase0(const Dev &d, hipStream_t s);
+void launch_phase1_detect(const Dev &d, hipStream_t s);
+void launch_phase1_mark(const Dev &d, hipStream_t s);
+void launch_apply1(const Dev &d, hipStream_t s);
+void launch_phase2(const Dev &d, int lanes, hipStream_t s);
+void launch_apply2(const Dev &d, hipStream_t s);
+void launch_final(const Dev &d, int *low1_out, int *ess_out, hipStream_t s);
+void launch_ctrl_init(const Dev &d, unsigned long long cap, hipStream_t s);
+void launch_reset_flags(const Dev &d, hipStream_t s);
+// Compaction, step 1: new offsets of the live columns in off2 and the live
+// total in ctrl->pool_top (and scan_tmp[ntiles]).
+void launch_compact_scan(const Dev &d, hipStream_t s);
+// Compaction, step 2: copy into pool2, then swap pool/pool2 and off/off2.
+void launch_compact_copy(Dev &d, hipStream_t s);
+
+}  // namespace oph

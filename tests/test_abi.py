@@ -1,0 +1,65 @@
+"""CPU tests of the C ABI library: it loads, exports every entry point that
+include/openph.h declares, and its host-side argument checks answer without
+touching a device."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import openph_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "openph.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(openph[a-z_0-9]*)\s*\(", text, re.M)
+    return sorted(set(names))
+
+
+def test_library_loads_and_exports_header_symbols():
+    L = openph_amd.lib()
+    names = declared_functions()
+    assert "openph" in names and "openph_v2" in names and "openph_pms_device" in names
+    for n in names:
+        assert hasattr(L, n), n
+    nm = os.popen(f"nm -D --defined-only {openph_amd._SO}").read()
+    for n in names:
+        assert re.search(rf"\bT {n}\b", nm), n
+
+
+def test_version_and_strerror():
+    L = openph_amd.lib()
+    assert L.openph_version().startswith(b"openph-mi355x")
+    for code in range(0, -9, -1):
+        assert L.openph_strerror(code)
+
+
+def test_unknown_algorithm_rejected_on_host():
+    with pytest.raises(openph_amd.OpenPHError) as e:
+        openph_amd.ph("not_an_algorithm", [1], [2], 2, 1, [0, 0])
+    assert e.value.status == -3
+
+
+def test_negative_m_rejected_on_host():
+    L = openph_amd.lib()
+    low = np.zeros(1, np.int32)
+    ess = np.zeros(1, np.int32)
+    rc = L.openph_v2(b"pms", None, None, -1, 0, None, None,
+                     low.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                     ess.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), None, None)
+    assert rc == -1
+
+
+def test_mismatched_coo_lengths_rejected():
+    with pytest.raises(openph_amd.OpenPHError):
+        openph_amd.ph("pms", [1, 2], [2], 3, 1, None)
+
+
+def test_library_has_gfx950_code_object():
+    blob = open(openph_amd._SO, "rb").read()
+    assert b"gfx950" in blob

@@ -1,0 +1,189 @@
+"""GPU parity tests: the HIP pms engine through the C ABI against the oracle.
+
+Bar: bit-exact low and ess. Where the oracle runs the same schedule (its
+snapshot mode publishes Phase II pivots after the pass, as the GPU does) the
+iteration count, the number of column additions and the merged elements
+must match too.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import openph_amd
+from openph_amd import complexes as C
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+INDEX = json.load(open(os.path.join(GOLD, "index.json")))
+sys.path.insert(0, os.path.dirname(__file__))
+from test_oracle import _gen, invariants  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    openph_amd.lib()
+
+
+@pytest.mark.parametrize("name", sorted(INDEX))
+def test_pms_matches_golden(name):
+    say("case", name)
+    f = np.load(os.path.join(GOLD, name + ".npz"))
+    b = _gen(name)
+    low, ess, st = openph_amd.reduce(b, "pms")
+    assert np.array_equal(low, f["low"])
+    assert np.array_equal(ess, f["ess"])
+    snap = INDEX[name]["pms_snapshot"]
+    assert st.num_iters == snap["num_iters"]
+    assert st.additions == snap["additions"]
+    assert st.merge_elems == snap["merge_elems"]
+    assert st.m == b.m
+
+
+@pytest.mark.parametrize("algo", openph_amd.ALGORITHMS)
+def test_every_algstr_gives_the_same_low(algo):
+    say("algo", algo)
+    f = np.load(os.path.join(GOLD, "vr40.npz"))
+    b = C.vr40()
+    r, c = b.coo1()
+    out = openph_amd.ph(algo, r, c, b.m, 8, np.zeros(b.m, np.int32))
+    assert len(out) == 8
+    low1, ess = out[0], out[1]
+    assert np.array_equal(low1, f["low"] + 1)
+    assert np.array_equal(ess, f["ess"])
+    assert out[7][0] >= 1
+
+
+@pytest.mark.parametrize("side", [24, 40])
+def test_cubical_vs_oracle(side):
+    say("cubical", side)
+    b = C.cubical64(seed=side, side=side)
+    ref = oracle.pms(b, snapshot=True)
+    low, ess, st = openph_amd.reduce(b, "pms")
+    assert np.array_equal(low, ref.low)
+    assert np.array_equal(ess, ref.ess)
+    assert st.additions == ref.additions
+    assert st.num_iters == ref.num_iters
+
+
+@pytest.mark.slow
+def test_cubical64_full_size():
+    say("cubical64")
+    b = C.cubical64()
+    ref = oracle.pms(b, snapshot=True)
+    low, ess, st = openph_amd.reduce(b, "pms")
+    assert np.array_equal(low, ref.low)
+    assert np.array_equal(ess, ref.ess)
+    # contractible grid: a single essential class
+    assert int(ess.sum()) == 1
+
+
+def test_device_entry_point_with_torch_buffers():
+    import torch
+    say("device entry")
+    b = C.torus100()
+    f = np.load(os.path.join(GOLD, "torus100.npz"))
+    r, c = b.coo1()
+    dr = torch.from_numpy(r).cuda()
+    dc = torch.from_numpy(c).cuda()
+    low = torch.full((b.m,), -7, dtype=torch.int32, device="cuda")
+    ess = torch.full((b.m,), -7, dtype=torch.int32, device="cuda")
+    for _ in range(3):  # workspace reuse across calls
+        st = openph_amd.pms_device(dr, dc, b.m, low, ess)
+        torch.cuda.synchronize()
+        assert np.array_equal(low.cpu().numpy(), f["low"] + 1)
+        assert np.array_equal(ess.cpu().numpy(), f["ess"])
+        assert st.num_iters == INDEX["torus100"]["pms_snapshot"]["num_iters"]
+
+
+def test_small_pool_compacts_and_stays_exact():
+    say("small pool")
+    b = C.torus100()
+    f = np.load(os.path.join(GOLD, "torus100.npz"))
+    opt = openph_amd.Options()
+    opt.pool_elems = b.nnz + 4096
+    low, ess, st = openph_amd.reduce(b, "pms", opt)
+    assert st.compactions > 0
+    assert np.array_equal(low, f["low"])
+    assert np.array_equal(ess, f["ess"])
+
+
+def test_pool_budget_too_small_is_an_error_not_a_wrong_answer():
+    say("pool budget")
+    b = C.torus100()
+    opt = openph_amd.Options()
+    opt.pool_elems = b.nnz + 1024
+    opt.pool_max_elems = b.nnz + 1024
+    with pytest.raises(openph_amd.OpenPHError) as e:
+        openph_amd.reduce(b, "pms", opt)
+    assert e.value.status == -4
+
+
+def test_sizes_in_a_row_reuse_the_workspace():
+    say("sizes in a row")
+    for name in ["vr40", "vr12", "torus100", "random1", "cubical8"]:
+        f = np.load(os.path.join(GOLD, name + ".npz"))
+        low, ess, _ = openph_amd.reduce(_gen(name), "pms")
+        assert np.array_equal(low, f["low"]), name
+        assert np.array_equal(ess, f["ess"]), name
+
+
+def test_edge_cases_empty_and_vertices_only():
+    say("edge cases")
+    b = C.Boundary(colptr=np.zeros(1, np.int64), rows=np.zeros(0, np.int32),
+                   dim=np.zeros(0, np.int32), value=np.zeros(0))
+    low, ess, st = openph_amd.reduce(b, "pms")
+    assert low.size == 0 and ess.size == 0
+    b = C.Boundary(colptr=np.zeros(6, np.int64), rows=np.zeros(0, np.int32),
+                   dim=np.zeros(5, np.int32), value=np.zeros(5))
+    low, ess, st = openph_amd.reduce(b, "pms")
+    assert (low == -1).all() and (ess == 1).all()
+
+
+@pytest.mark.parametrize("kind", ["row_ge_col", "unsorted_cols", "unsorted_rows", "out_of_range"])
+def test_malformed_input_is_rejected(kind):
+    say("malformed", kind)
+    # a triangle: vertices 1..3, edges 4..6, face 7
+    r = np.array([1, 2, 1, 3, 2, 3, 4, 5, 6], np.int32)
+    c = np.array([4, 4, 5, 5, 6, 6, 7, 7, 7], np.int32)
+    if kind == "row_ge_col":
+        r = r.copy(); r[0] = 4
+    elif kind == "unsorted_cols":
+        c = c.copy(); c[2], c[6] = c[6], c[2]
+    elif kind == "unsorted_rows":
+        r = r.copy(); r[0], r[1] = r[1], r[0]
+    else:
+        r = r.copy(); r[-1] = 99
+    with pytest.raises(openph_amd.OpenPHError) as e:
+        openph_amd.ph("pms", r, c, 7, 1, None)
+    assert e.value.status in (-1, -2)
+    # the same call with the valid matrix still works afterwards
+    r0 = np.array([1, 2, 1, 3, 2, 3, 4, 5, 6], np.int32)
+    c0 = np.array([4, 4, 5, 5, 6, 6, 7, 7, 7], np.int32)
+    out = openph_amd.ph("pms", r0, c0, 7, 1, None)
+    assert out[0].tolist() == [0, 0, 0, 2, 3, 0, 6]
+    assert out[1].tolist() == [1, 0, 0, 0, 0, 0, 0]
+
+
+def test_dropin_openph_matches_openph_m_fields():
+    say("openph()")
+    b = C.vr40()
+    r, c = b.coo1()
+    out, t = openph_amd.openph(r, c, b.m, "pms", 8, np.zeros(b.m, np.int32))
+    f = np.load(os.path.join(GOLD, "vr40.npz"))
+    assert np.array_equal(out["low"], f["low"] + 1)
+    for k in ("ess", "err_linf", "err_lone", "err_redu", "err_ess", "time_track",
+              "num_iters", "time_inner", "m"):
+        assert k in out
+    assert t > 0
