@@ -89,6 +89,7 @@ typedef struct openph_stats {
     float phase2_ms;         /* summed device time of the Phase II launches  */
     int phase2_launches;     /* number of Phase II launches                  */
     int m;                   /* columns reduced                              */
+    int overflows;           /* Phase II passes cut short by a full pool     */
 } openph_stats;
 
 /*

@@ -67,6 +67,7 @@ class Stats(ctypes.Structure):
         ("phase2_ms", ctypes.c_float),
         ("phase2_launches", ctypes.c_int),
         ("m", ctypes.c_int),
+        ("overflows", ctypes.c_int),
     ]
 
     def as_dict(self):

@@ -133,7 +133,7 @@ struct RunCfg {
 
 static RunCfg cfg_from(const openph_options *opt, int nnz) {
     RunCfg r;
-    size_t def = (size_t)nnz * 8 + (1u << 22);
+    size_t def = std::max<size_t>((size_t)nnz * 16, (size_t)1 << 26);
     r.pool_elems = def;
     if (opt) {
         if (opt->max_iters > 0) r.max_iters = opt->max_iters;
@@ -156,13 +156,13 @@ static int err_from_bits(int bits) {
 // Pool compaction, growing the pool when the live data fills more than half
 // of it. Returns OPENPH_ENOMEM when the pool budget cannot hold the live data
 // with room to make progress.
-static int compact(Context &c, const RunCfg &cfg, hipStream_t s, int *ncompact) {
+static int compact(Context &c, const RunCfg &cfg, hipStream_t s, int *ncompact, size_t &lcap) {
     Dev &d = c.d;
     launch_compact_scan(d, s);
     unsigned long long live = 0;
     OPH_CHECK(hipMemcpyAsync(&live, &d.ctrl->pool_top, sizeof(live), hipMemcpyDeviceToHost, s));
     OPH_CHECK(hipStreamSynchronize(s));
-    size_t cap = c.cap_pool;
+    size_t cap = lcap;
     size_t want = cap;
     if (live * 2 > cap) want = std::max<size_t>(cap * 2, live * 2);
     if (want > cfg.pool_max) want = cfg.pool_max;
@@ -170,8 +170,9 @@ static int compact(Context &c, const RunCfg &cfg, hipStream_t s, int *ncompact) 
     OPH_CHECK(grow(&d.pool2, c.cap_pool2, want));
     launch_compact_copy(d, s);
     std::swap(c.cap_pool, c.cap_pool2);
-    // the pool the kernels now use has capacity c.cap_pool
-    unsigned long long capv = c.cap_pool;
+    // the kernels now use pool2's buffer, logically `want` entries long
+    lcap = want;
+    unsigned long long capv = lcap;
     OPH_CHECK(hipMemcpyAsync(&d.ctrl->pool_cap, &capv, sizeof(capv), hipMemcpyHostToDevice, s));
     (*ncompact)++;
     return OPENPH_OK;
@@ -199,7 +200,10 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
         OPH_CHECK(hipEventCreate(&c.ev0));
         OPH_CHECK(hipEventCreate(&c.ev1));
     }
-    launch_ctrl_init(d, (unsigned long long)c.cap_pool, s);
+    // logical pool size: the requested one, even when the context holds a
+    // larger buffer from an earlier call
+    size_t lcap = std::min(c.cap_pool, std::max(cfg.pool_elems, (size_t)nnz));
+    launch_ctrl_init(d, (unsigned long long)lcap, s);
     launch_setup(d, rows1, cols1, s);
     Ctrl h{};
     OPH_CHECK(hipMemcpyAsync(&h, d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
@@ -212,10 +216,11 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
     OPH_CHECK(hipEventRecord(c.ev0, s));
     launch_phase0(d, s);
 
-    int ncompact = 0;
+    int ncompact = 0, noverflow = 0;
     unsigned long long peak = nnz;
     unsigned long long prev_add = 0;
     int prev_marks = 0;
+    int stalls = 0;
     int iter = 1;
     bool converged = false;
     if (time_track && c.iter_events.size() < 2) {
@@ -275,11 +280,17 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
         // pms makes progress every iteration while lows repeat (the first
         // occurrence of the largest repeated low is marked, or its repeats
         // are reduced); no progress without an overflow means a fault.
-        if (!h.overflow && h.additions == prev_add && h.marks == prev_marks) return OPENPH_EITER;
+        bool progress = h.additions != prev_add || h.marks != prev_marks;
+        if (!h.overflow && !progress) return OPENPH_EITER;
+        // a full pool with no progress twice in a row (the second time right
+        // after a compaction) means the pool budget cannot hold the work
+        stalls = (h.overflow && !progress) ? stalls + 1 : 0;
+        if (stalls >= 2) return OPENPH_ENOMEM;
         prev_add = h.additions;
         prev_marks = h.marks;
-        if (h.overflow || h.pool_top * 4 > (unsigned long long)c.cap_pool * 3) {
-            rc = compact(c, cfg, s, &ncompact);
+        if (h.overflow) noverflow++;
+        if (h.overflow || h.pool_top * 4 > (unsigned long long)lcap * 3) {
+            rc = compact(c, cfg, s, &ncompact, lcap);
             if (rc) return rc;
         }
         iter++;
@@ -292,6 +303,7 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
     if (stats) {
         stats->num_iters = iter;
         stats->compactions = ncompact;
+        stats->overflows = noverflow;
         stats->additions = (int64_t)h.additions;
         stats->merge_elems = (int64_t)h.merge_elems;
         stats->pool_peak = (int64_t)std::max(peak, h.pool_top);
