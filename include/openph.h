@@ -73,9 +73,15 @@ typedef struct openph_options {
                                 (0 = automatic)                              */
     int64_t pool_max_elems;  /* largest pool the reduction may grow to
                                 (0 = 2^32 - 1 entries, 16 GiB)               */
-    int lanes_per_column;    /* Phase II lanes per column: 0 = autotuned,
-                                else 1, 2, 4, 8, 16, 32 or 64                */
+    int lanes_per_column;    /* Phase II lanes per column: 0 = automatic,
+                                else 1, 4, 8, 16, 32 or 64                   */
     int track;               /* 1 = fill the per-iteration trackers          */
+    int slot_elems;          /* minimum pool segment per column (0 = 8)      */
+    int group_lds;           /* LDS entries per lane-group buffer (0 = the
+                                default of lanes_per_column); the compiled
+                                shapes are {4: 32, 64}, {8: 32, 64, 128},
+                                {16: 64, 128, 256}, {32: 256, 512},
+                                {64: 512, 1024}                              */
 } openph_options;
 
 /* Statistics of one reduction. */
@@ -90,6 +96,9 @@ typedef struct openph_stats {
     int phase2_launches;     /* number of Phase II launches                  */
     int m;                   /* columns reduced                              */
     int overflows;           /* Phase II passes cut short by a full pool     */
+    int lanes;               /* Phase II lanes per column that were used     */
+    int spills;              /* columns whose additions left LDS             */
+    int group_lds;           /* LDS entries per lane-group buffer used       */
 } openph_stats;
 
 /*

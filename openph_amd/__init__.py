@@ -52,6 +52,8 @@ class Options(ctypes.Structure):
         ("pool_max_elems", ctypes.c_int64),
         ("lanes_per_column", ctypes.c_int),
         ("track", ctypes.c_int),
+        ("slot_elems", ctypes.c_int),
+        ("group_lds", ctypes.c_int),
     ]
 
 
@@ -68,6 +70,9 @@ class Stats(ctypes.Structure):
         ("phase2_launches", ctypes.c_int),
         ("m", ctypes.c_int),
         ("overflows", ctypes.c_int),
+        ("lanes", ctypes.c_int),
+        ("spills", ctypes.c_int),
+        ("group_lds", ctypes.c_int),
     ]
 
     def as_dict(self):

@@ -9,7 +9,7 @@ namespace oph {
 
 constexpr int kMaxSlots = 16;      // distinct pseudo-dimensions supported
 constexpr int kBlock = 256;        // threads per block for flat passes
-constexpr int kItems = 4;          // columns per thread in the Phase I scan
+constexpr int kItems = 4;          // columns per thread in the tiled scans
 constexpr int kTile = kBlock * kItems;
 constexpr int kIntMax = 0x7fffffff;
 
@@ -20,13 +20,14 @@ struct Ctrl {
     unsigned long long pool_cap;   // pool capacity (entries)
     unsigned long long additions;  // column additions performed
     unsigned long long merge_elems;// sum |a|+|b|+|a xor b|
-    int err;                       // validation error bits
+    int err;                       // validation / consistency error bits
     int any_dup;                   // a non-negative low occurs twice
     int overflow;                  // Phase II ran out of pool
     int nslots;                    // distinct pseudo-dimensions
     int maxlen;                    // longest input column
     int marks;                     // columns marked by Phase I (cumulative)
-    int pad[2];
+    int nwork;                     // columns queued for Phase II
+    int spills;                    // Phase II columns that left LDS (cumulative)
 };
 
 // Validation error bits (Ctrl::err).
@@ -43,12 +44,15 @@ struct Dev {
     int m = 0;
     int nnz = 0;
     int ntiles = 0;
+    int slot0 = 8;      // minimum segment size of a column in the pool
     // per column
     int *low = nullptr, *arglow = nullptr, *classes = nullptr, *ess = nullptr;
     int *left = nullptr, *beta = nullptr, *first = nullptr, *clear = nullptr;
-    int *newpiv = nullptr, *slot = nullptr, *len = nullptr;
+    int *newpiv = nullptr, *slot = nullptr, *len = nullptr, *cap = nullptr;
+    int *work = nullptr;                     // Phase II worklist
+    int4 *pivdesc = nullptr;                 // per row: {pivot column, off, len, 0}
     unsigned *off = nullptr, *off2 = nullptr;
-    unsigned long long *scan_tmp = nullptr;  // compaction scan partials
+    unsigned long long *scan_tmp = nullptr;  // tiled-scan partials
     int *slotmap = nullptr;                  // pseudo-dim + 1 -> slot
     int *tile_max = nullptr, *tile_pre = nullptr;
     int *pool = nullptr, *pool2 = nullptr;   // column pool (int32 rows)
@@ -56,17 +60,24 @@ struct Dev {
 };
 
 // ---- launchers (kernels.hip) ----
+void launch_ctrl_init(const Dev &d, unsigned long long cap, hipStream_t s);
+// COO -> per-column state and pool layout (slot-granular segments).
 void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t s);
 void launch_slots(const Dev &d, hipStream_t s);
 void launch_phase0(const Dev &d, hipStream_t s);
+void launch_reset_flags(const Dev &d, hipStream_t s);
 void launch_phase1_detect(const Dev &d, hipStream_t s);
 void launch_phase1_mark(const Dev &d, hipStream_t s);
 void launch_apply1(const Dev &d, hipStream_t s);
-void launch_phase2(const Dev &d, int lanes, hipStream_t s);
+// lanes: 1 = one lane per column over all columns; 4..64 = lane groups over
+// the worklist with wlds LDS entries per buffer. grid = blocks for the group
+// kernels.
+void launch_phase2(const Dev &d, int lanes, int wlds, int grid, hipStream_t s);
+bool phase2_shape_ok(int lanes, int wlds);
+int phase2_default_lds(int lanes);
+int phase2_blocks_per_cu(int lanes, int wlds);
 void launch_apply2(const Dev &d, hipStream_t s);
 void launch_final(const Dev &d, int *low1_out, int *ess_out, hipStream_t s);
-void launch_ctrl_init(const Dev &d, unsigned long long cap, hipStream_t s);
-void launch_reset_flags(const Dev &d, hipStream_t s);
 // Compaction, step 1: new offsets of the live columns in off2 and the live
 // total in ctrl->pool_top (and scan_tmp[ntiles]).
 void launch_compact_scan(const Dev &d, hipStream_t s);

@@ -8,10 +8,15 @@
 //   ess[j]     1 while cell j may be essential
 //   left[i]    leftmost column holding row i (INT_MAX if none), beta.cu:2-15
 //   beta[j]    max{i : left[i] = j}, beta.cu:2-15
+// and, new here:
+//   pivdesc[i] {arglow[i], off, len, 0} of the pivot column of row i, so a
+//              column addition finds its operand with one 16-byte load
+//   work[]     the columns Phase II has to reduce in this iteration
 // Columns live in a pool of int32 row indices: column j occupies
-// pool[off[j] .. off[j]+len[j]) in ascending order. A column addition
-// writes its result to a fresh pool segment (bump allocation), so the
-// previous contents stay valid until the host compacts the pool.
+// pool[off[j] .. off[j]+len[j]) in ascending order inside a segment of
+// cap[j] >= len[j] entries (at least slot0), so most Phase II results are
+// written back in place; a column that outgrows its segment gets a new one
+// from a bump allocator, and the host compacts the pool when it fills.
 #include "engine.h"
 
 namespace oph {
@@ -77,6 +82,108 @@ __device__ __forceinline__ unsigned long long wave_alloc(unsigned long long *top
     return base + (incl - v);
 }
 
+// Block-aggregated append to a list, in thread order inside the block: one
+// global atomic per block. Every thread of the block calls it.
+__device__ __forceinline__ void block_append(int *list, int *count, int value, bool take) {
+    __shared__ int wcnt[kBlock / 64];
+    __shared__ int wbase[kBlock / 64];
+    __shared__ int bbase;
+    const unsigned long long b = __ballot(take);
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
+    if (lane == 0) wcnt[wid] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+            wbase[w] = tot;
+            tot += wcnt[w];
+        }
+        bbase = tot ? atomicAdd(count, tot) : 0;
+    }
+    __syncthreads();
+    if (take) list[bbase + wbase[wid] + __popcll(b & ((1ull << lane) - 1))] = value;
+}
+
+__device__ __forceinline__ unsigned slot_size(int len, int slot0) {
+    return (unsigned)(len > slot0 ? len : slot0);
+}
+
+// ---------------------------------------------------------------------------
+// Tiled exclusive scan of per-column sizes (pool layout).
+//   mode 0: setup layout, size = max(len, slot0)
+//   mode 1: compaction, size = 0 for cleared columns (never read again),
+//           len for final pivots (never written again), max(len, slot0) else
+// Offsets go to `out`; the total to scan_tmp[ntiles] and ctrl->pool_top.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ unsigned col_size(const Dev &d, int j, int mode) {
+    if (j >= d.m) return 0u;
+    int n = d.len[j];
+    if (mode == 0) return slot_size(n, d.slot0);
+    int c = d.classes[j];
+    if (c == 1) return 0u;
+    if (c == -1) return (unsigned)n;
+    return slot_size(n, d.slot0);
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(Dev d, int mode) {
+    __shared__ unsigned long long lds[kBlock / 64];
+    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
+    unsigned long long sum = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; k++) sum += col_size(d, base + k, mode);
+    unsigned long long tot;
+    block_exclusive_scan(sum, SumOp<unsigned long long>(), 0ull, lds, &tot);
+    if (threadIdx.x == 0) d.scan_tmp[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_partials(Dev d) {
+    __shared__ unsigned long long lds[16];
+    unsigned long long running = 0;
+    for (int c0 = 0; c0 < d.ntiles; c0 += blockDim.x) {
+        int t = c0 + threadIdx.x;
+        unsigned long long v = t < d.ntiles ? d.scan_tmp[t] : 0ull;
+        unsigned long long tot;
+        unsigned long long ex = block_exclusive_scan(v, SumOp<unsigned long long>(), 0ull, lds, &tot);
+        if (t < d.ntiles) d.scan_tmp[t] = ex + running;
+        running += tot;
+    }
+    if (threadIdx.x == 0) {
+        d.scan_tmp[d.ntiles] = running;
+        d.ctrl->pool_top = running;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_apply(Dev d, int mode, unsigned *out) {
+    __shared__ unsigned long long lds[kBlock / 64];
+    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
+    unsigned v[kItems];
+    unsigned long long sum = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        v[k] = col_size(d, base + k, mode);
+        sum += v[k];
+    }
+    unsigned long long tot;
+    unsigned long long ex = block_exclusive_scan(sum, SumOp<unsigned long long>(), 0ull, lds, &tot);
+    ex += d.scan_tmp[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        int j = base + k;
+        if (j < d.m) {
+            out[j] = (unsigned)ex;
+            if (mode == 0) d.cap[j] = (int)v[k];
+        }
+        ex += v[k];
+    }
+}
+
+static void launch_scan(const Dev &d, int mode, unsigned *out, hipStream_t s) {
+    k_scan_tiles<<<d.ntiles, kBlock, 0, s>>>(d, mode);
+    k_scan_partials<<<1, 1024, 0, s>>>(d);
+    k_scan_apply<<<d.ntiles, kBlock, 0, s>>>(d, mode, out);
+}
+
 // ---------------------------------------------------------------------------
 // Setup: COO (1-based) -> pool + per-column state
 // ---------------------------------------------------------------------------
@@ -93,7 +200,13 @@ __global__ void k_ctrl_init(Ctrl *c, unsigned long long cap) {
         c->nslots = 0;
         c->maxlen = 0;
         c->marks = 0;
+        c->nwork = 0;
+        c->spills = 0;
     }
+}
+
+void launch_ctrl_init(const Dev &d, unsigned long long cap, hipStream_t s) {
+    k_ctrl_init<<<1, 64, 0, s>>>(d.ctrl, cap);
 }
 
 __device__ __forceinline__ int lower_bound_i(const int *a, int n, int key) {
@@ -107,7 +220,8 @@ __device__ __forceinline__ int lower_bound_i(const int *a, int n, int key) {
 }
 
 // One thread per column: column extent by binary search in the sorted cols,
-// initial state (openph.cu:86-90 fills, kernels.cu:40-52 compute_low).
+// initial state (openph.cu:86-90 fills, kernels.cu:40-52 compute_low). The
+// COO start of the column goes to off2 until the pool layout is known.
 __global__ void k_init_cols(Dev d, const int *__restrict__ rows1, const int *__restrict__ cols1) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= d.m) return;
@@ -115,7 +229,7 @@ __global__ void k_init_cols(Dev d, const int *__restrict__ rows1, const int *__r
     int hi = lower_bound_i(cols1, d.nnz, j + 2);
     if (hi < lo) hi = lo;  // unsorted input; flagged by k_entries
     int n = hi - lo;
-    d.off[j] = (unsigned)lo;
+    d.off2[j] = (unsigned)lo;
     d.len[j] = n;
     int l = n ? rows1[hi - 1] - 1 : -1;
     if (l < -1 || l >= d.m) l = -1;  // out of range; flagged by k_entries
@@ -128,24 +242,54 @@ __global__ void k_init_cols(Dev d, const int *__restrict__ rows1, const int *__r
     d.first[j] = kIntMax;
     d.clear[j] = 0;
     d.newpiv[j] = -1;
+    d.pivdesc[j] = make_int4(-1, 0, 0, 0);
     if (n) atomicMax(&d.ctrl->maxlen, n);
 }
 
-// One thread per entry: validation, pool copy (0-based), left[] by atomicMin.
-__global__ void k_entries(Dev d, const int *__restrict__ rows1, const int *__restrict__ cols1) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= d.nnz) return;
-    int c = cols1[e] - 1, r = rows1[e] - 1;
+// Validation, pool copy (0-based) and left[] by atomicMin, kTile entries per
+// block. A row occurs in many columns (an edge in every triangle on it), so
+// the minimum column per row is first taken in an LDS hash table and then
+// issued once per distinct row and block.
+__global__ void __launch_bounds__(kBlock) k_entries(Dev d, const int *__restrict__ rows1,
+                                                   const int *__restrict__ cols1) {
+    constexpr int H = 2 * kTile;
+    __shared__ int hkey[H];
+    __shared__ int hval[H];
+    for (int i = threadIdx.x; i < H; i += kBlock) {
+        hkey[i] = -1;
+        hval[i] = kIntMax;
+    }
+    __syncthreads();
     int bad = 0;
-    if (c < 0 || c >= d.m || r < 0 || r >= c) bad |= kErrRange;
-    if (e > 0) {
-        int pc = cols1[e - 1] - 1;
-        if (c < pc) bad |= kErrOrder;
-        else if (c == pc && rows1[e - 1] - 1 >= r) bad |= kErrRows;
+    for (int k = 0; k < kItems; k++) {
+        const int e = blockIdx.x * kTile + k * kBlock + threadIdx.x;  // coalesced
+        if (e >= d.nnz) continue;
+        const int c = cols1[e] - 1, r = rows1[e] - 1;
+        int b = 0;
+        if (c < 0 || c >= d.m || r < 0 || r >= c) b |= kErrRange;
+        if (e > 0) {
+            const int pc = cols1[e - 1] - 1;
+            if (c < pc) b |= kErrOrder;
+            else if (c == pc && rows1[e - 1] - 1 >= r) b |= kErrRows;
+        }
+        bad |= b;
+        if (b & kErrRange) continue;
+        const int q = e - (int)d.off2[c];
+        if (q >= 0 && q < d.len[c]) d.pool[d.off[c] + q] = r;
+        unsigned h = ((unsigned)r * 2654435761u) >> 21;
+        while (true) {
+            const int old = atomicCAS(&hkey[h], -1, r);
+            if (old == -1 || old == r) break;
+            h = (h + 1) & (H - 1);
+        }
+        atomicMin(&hval[h], c);
     }
     if (bad) atomicOr(&d.ctrl->err, bad);
-    d.pool[e] = r;
-    if (!(bad & kErrRange)) atomicMin(&d.left[r], c);
+    __syncthreads();
+    for (int i = threadIdx.x; i < H; i += kBlock) {
+        const int key = hkey[i];
+        if (key > -1) atomicMin(&d.left[key], hval[i]);
+    }
 }
 
 __global__ void k_beta(Dev d) {
@@ -156,8 +300,9 @@ __global__ void k_beta(Dev d) {
 }
 
 void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t s) {
-    int gm = (d.m + kBlock - 1) / kBlock, ge = (d.nnz + kBlock - 1) / kBlock;
-    if (d.m) k_init_cols<<<gm, kBlock, 0, s>>>(d, rows1, cols1);
+    int gm = (d.m + kBlock - 1) / kBlock, ge = (d.nnz + kTile - 1) / kTile;
+    k_init_cols<<<gm, kBlock, 0, s>>>(d, rows1, cols1);
+    launch_scan(d, 0, d.off, s);
     if (d.nnz) k_entries<<<ge, kBlock, 0, s>>>(d, rows1, cols1);
 }
 
@@ -202,6 +347,12 @@ void launch_slots(const Dev &d, hipStream_t s) {
     k_assign_slot<<<gm, kBlock, 0, s>>>(d);
 }
 
+// Publishes column j as the pivot of row l.
+__device__ __forceinline__ void publish_pivot(const Dev &d, int l, int j) {
+    d.arglow[l] = j;
+    d.pivdesc[l] = make_int4(j, (int)d.off[j], d.len[j], 0);
+}
+
 // ---------------------------------------------------------------------------
 // Phase 0: mark_pivots_and_clear (pms_functions.cu:33-53)
 // ---------------------------------------------------------------------------
@@ -212,24 +363,35 @@ __global__ void k_phase0(Dev d) {
     int lj = d.low[j];
     if (lj > -1 && lj == d.beta[j]) {
         d.classes[j] = -1;
-        d.arglow[lj] = j;
+        publish_pivot(d, lj, j);
         d.clear[lj] = 1;
         d.ess[lj] = 0;
         d.ess[j] = 0;
     }
 }
 
-// clear_positives (pms_functions.cu:131-140) and reset of the first-occurrence
-// table for the next Phase I.
+// clear_positives (pms_functions.cu:131-140), reset of the first-occurrence
+// table and construction of the Phase II worklist: columns still unknown
+// whose low has a pivot to their left.
 __global__ void k_apply1(Dev d) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= d.m) return;
-    if (d.clear[t]) {
-        d.low[t] = -1;
-        d.classes[t] = 1;
-        d.clear[t] = 0;
+    bool take = false;
+    if (t < d.m) {
+        int c = d.classes[t], lt = d.low[t];
+        if (d.clear[t]) {
+            lt = -1;
+            c = 1;
+            d.low[t] = -1;
+            d.classes[t] = 1;
+            d.clear[t] = 0;
+        }
+        d.first[t] = kIntMax;
+        if (c == 0 && lt > -1) {
+            int p = d.arglow[lt];
+            take = p > -1 && p < t;
+        }
     }
-    d.first[t] = kIntMax;
+    block_append(d.work, &d.ctrl->nwork, t, take);
 }
 
 void launch_phase0(const Dev &d, hipStream_t s) {
@@ -243,6 +405,18 @@ void launch_apply1(const Dev &d, hipStream_t s) {
     k_apply1<<<gm, kBlock, 0, s>>>(d);
 }
 
+__global__ void k_reset_flags(Ctrl *c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c->any_dup = 0;
+        c->overflow = 0;
+        c->nwork = 0;
+    }
+}
+
+void launch_reset_flags(const Dev &d, hipStream_t s) {
+    k_reset_flags<<<1, 64, 0, s>>>(d.ctrl);
+}
+
 // ---------------------------------------------------------------------------
 // Phase I: lowstar detection (pms_functions.cu:55-85) as a segmented scan.
 //
@@ -252,13 +426,67 @@ void launch_apply1(const Dev &d, hipStream_t s) {
 // pseudo-dimension (-1 if none). j is marked when it is the first occurrence
 // of L, classes[j] == 0 and L > ceil(j). Rows of different pseudo-dimensions
 // are disjoint, so first[] can be global.
+//
+// first[] needs atomics only for unknown columns whose low has no pivot yet:
+// a final pivot p of row L is the first column with low L (no column left of
+// p can ever reach low L: p is the first column whose span holds a vector
+// with low L), and every other column with low L is therefore a duplicate.
+// This keeps the many columns that share a pivot's low off the atomic.
 // ---------------------------------------------------------------------------
 
-__global__ void k_first(Dev d) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= d.m) return;
-    int lj = d.low[j];
-    if (lj > -1) atomicMin(&d.first[lj], j);
+// Columns that share a low tend to be neighbours in filtration order (VR
+// simplices born at the same diameter, cubical cells of one vertex), and
+// device-scope atomics on one address serialize. Each block therefore first
+// takes the minimum per distinct low of its tile in an LDS hash table and
+// then issues one global atomicMin per distinct low.
+constexpr int kHash = 2 * kTile;  // open addressing, load factor <= 1/2
+
+__global__ void __launch_bounds__(kBlock) k_first(Dev d) {
+    __shared__ int hkey[kHash];
+    __shared__ int hval[kHash];
+    for (int i = threadIdx.x; i < kHash; i += kBlock) {
+        hkey[i] = -1;
+        hval[i] = kIntMax;
+    }
+    __syncthreads();
+    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        const int j = base + k;
+        if (j >= d.m) continue;
+        const int lj = d.low[j];
+        if (lj < 0 || d.classes[j] != 0 || d.arglow[lj] > -1) continue;
+        unsigned h = ((unsigned)lj * 2654435761u) >> 21;  // 11 bits = kHash
+        while (true) {
+            const int old = atomicCAS(&hkey[h], -1, lj);
+            if (old == -1 || old == lj) break;
+            h = (h + 1) & (kHash - 1);
+        }
+        atomicMin(&hval[h], j);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHash; i += kBlock) {
+        const int key = hkey[i];
+        if (key > -1) atomicMin(&d.first[key], hval[i]);
+    }
+}
+static_assert(kHash == 2048, "k_first hashes to 11 bits");
+
+// dup = L when column j repeats an earlier column's low L, else -1;
+// fst = 1 when j is the first column with its low.
+__device__ __forceinline__ void dup_state(const Dev &d, int j, int lj, int &dup, int &fst) {
+    dup = -1;
+    fst = 0;
+    if (lj < 0) return;
+    const int p = d.arglow[lj];
+    if (p > -1) {
+        if (p == j) fst = 1;
+        else dup = lj;
+        return;
+    }
+    const int f = d.first[lj];
+    if (f == j) fst = 1;
+    else if (f < j) dup = lj;
 }
 
 // Per tile of kTile columns: max duplicate low per slot, and the global
@@ -275,9 +503,10 @@ __global__ void __launch_bounds__(kBlock) k_tile(Dev d) {
     for (int k = 0; k < kItems; k++) {
         int j = base + k;
         if (j < d.m) {
-            int lj = d.low[j];
-            if (lj > -1 && d.first[lj] < j) {
-                atomicMax(&smax[d.slot[j]], lj);
+            int dup, fst;
+            dup_state(d, j, d.low[j], dup, fst);
+            if (dup > -1) {
+                atomicMax(&smax[d.slot[j]], dup);
                 dupseen = 1;
             }
         }
@@ -321,11 +550,7 @@ __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
         if (j < d.m) {
             lo[k] = d.low[j];
             sl[k] = d.slot[j];
-            if (lo[k] > -1) {
-                int f = d.first[lo[k]];
-                fst[k] = (f == j);
-                if (f < j) dup[k] = lo[k];
-            }
+            dup_state(d, j, lo[k], dup[k], fst[k]);
         }
     }
 #pragma unroll
@@ -353,7 +578,7 @@ __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
         if (d.classes[j] == 0 && lo[k] > c) {
             int L = lo[k];
             atomicAdd(&d.ctrl->marks, 1);
-            d.arglow[L] = j;
+            publish_pivot(d, L, j);
             d.classes[j] = -1;
             d.clear[L] = 1;
             d.ess[L] = 0;
@@ -363,8 +588,7 @@ __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
 }
 
 void launch_phase1_detect(const Dev &d, hipStream_t s) {
-    int gm = (d.m + kBlock - 1) / kBlock;
-    k_first<<<gm, kBlock, 0, s>>>(d);
+    k_first<<<d.ntiles, kBlock, 0, s>>>(d);
     k_tile<<<d.ntiles, kBlock, 0, s>>>(d);
 }
 
@@ -378,11 +602,12 @@ void launch_phase1_mark(const Dev &d, hipStream_t s) {
 // pivots known at the start of the phase.
 //
 // Pivots found here by the left rule (left[low_j] == j, the column is final)
-// go to newpiv[] and are published to arglow[] after the phase, so every
-// arglow[] entry read here names a column that is final and not written
-// during the phase. clear[] / ess[] writes only ever store 1 / 0 and are
-// facts about the matrix (a non-negative low of a partially reduced column
-// is a positive, paired index), so their order does not matter.
+// go to newpiv[] and are published after the phase, so every pivdesc[] entry
+// read here names a column that is final and not written during the phase.
+// clear[] / ess[] writes only ever store 1 / 0 and are facts about the matrix
+// (a non-negative low of a partially reduced column is a positive, paired
+// index), so their order does not matter. A column's own state (off, len,
+// cap, low, classes, newpiv) changes only once its data is in the pool.
 // ---------------------------------------------------------------------------
 
 // c = a xor b for ascending lists; returns |c|.
@@ -401,6 +626,8 @@ __device__ __forceinline__ int merge_xor(const int *__restrict__ a, int na,
     return n;
 }
 
+// One lane per column over all m columns, every addition into a fresh pool
+// segment. Kept as the lanes = 1 configuration of the autotuner.
 __global__ void __launch_bounds__(kBlock) k_phase2_lane(Dev d) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     bool active = false;
@@ -418,7 +645,7 @@ __global__ void __launch_bounds__(kBlock) k_phase2_lane(Dev d) {
         coff = d.off[j];
         clen = d.len[j];
     }
-    bool dirty = false, ovf = false;
+    bool dirty = false, ovf = false, becomes_pivot = false;
     unsigned long long nadd = 0, nel = 0;
     const unsigned long long cap = d.ctrl->pool_cap;
     // wave-uniform loop so that wave_alloc sees every lane
@@ -447,20 +674,16 @@ __global__ void __launch_bounds__(kBlock) k_phase2_lane(Dev d) {
                 int prev = lj;
                 lj = n ? dst[n - 1] : -1;
                 if (lj >= prev) {
-                    // the pivot's stored column does not end in its low:
-                    // inconsistent state, stop instead of looping
                     atomicOr(&d.ctrl->err, kErrState);
                     active = false;
                 } else if (lj < 0) {
                     active = false;
                 } else {
-                    if (d.left[lj] == j) {
-                        d.newpiv[lj] = j;
-                        d.classes[j] = -1;
-                        active = false;
-                    }
                     d.clear[lj] = 1;
-                    if (active) {
+                    if (d.left[lj] == j) {
+                        becomes_pivot = true;
+                        active = false;
+                    } else {
                         p = d.arglow[lj];
                         active = (p > -1 && p < j);
                     }
@@ -471,10 +694,14 @@ __global__ void __launch_bounds__(kBlock) k_phase2_lane(Dev d) {
     if (dirty) {
         d.off[j] = coff;
         d.len[j] = clen;
+        d.cap[j] = clen;
         d.low[j] = lj;
+        if (becomes_pivot) {
+            d.newpiv[lj] = j;
+            d.classes[j] = -1;
+        }
     }
     if (ovf) d.ctrl->overflow = 1;
-    // wave-reduce the counters, one atomic per wave
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         nadd += __shfl_down(nadd, o, 64);
@@ -486,10 +713,294 @@ __global__ void __launch_bounds__(kBlock) k_phase2_lane(Dev d) {
     }
 }
 
-void launch_phase2(const Dev &d, int lanes, hipStream_t s) {
-    (void)lanes;
-    int gm = (d.m + kBlock - 1) / kBlock;
-    k_phase2_lane<<<gm, kBlock, 0, s>>>(d);
+// Lane-group merge. G lanes of one wave own one column; its current contents
+// sit in LDS (sB), the pivot column is staged into sA, and the result goes to
+// sC. Rank form of the sorted symmetric difference: an element x of A that is
+// not in B lands at i + #{b < x} - 2 #{common < x}; #{common < x} is the number
+// of A-elements before x that were found in B, a prefix count taken with a
+// ballot per chunk of G elements (and symmetrically for B).
+template <int G>
+__device__ __forceinline__ unsigned long long group_bits(unsigned long long b, int gw) {
+    if constexpr (G == 64) return b;
+    else return (b >> (gw * G)) & ((1ull << G) - 1ull);
+}
+
+__device__ __forceinline__ int lds_lower_bound(const int *s, int n, int x) {
+    int lo = 0;
+    while (n > 0) {
+        int half = n >> 1;
+        if (s[lo + half] < x) {
+            lo += half + 1;
+            n -= half + 1;
+        } else {
+            n = half;
+        }
+    }
+    return lo;
+}
+
+template <int G>
+__device__ int group_xor(const int *sA, int na, const int *sB, int nb, int *sC, int gl, int gw) {
+    const unsigned long long below = (1ull << gl) - 1ull;
+    int common = 0;
+    for (int base = 0; base < na; base += G) {
+        const int i = base + gl;
+        const bool valid = i < na;
+        int x = 0, r = 0;
+        bool f = false;
+        if (valid) {
+            x = sA[i];
+            r = lds_lower_bound(sB, nb, x);
+            f = r < nb && sB[r] == x;
+        }
+        const unsigned long long bm = group_bits<G>(__ballot(f), gw);
+        const int pre = __popcll(bm & below);
+        if (valid && !f) sC[i + r - 2 * (common + pre)] = x;
+        common += __popcll(bm);
+    }
+    int common2 = 0;
+    for (int base = 0; base < nb; base += G) {
+        const int k = base + gl;
+        const bool valid = k < nb;
+        int y = 0, r = 0;
+        bool f = false;
+        if (valid) {
+            y = sB[k];
+            r = lds_lower_bound(sA, na, y);
+            f = r < na && sA[r] == y;
+        }
+        const unsigned long long bm = group_bits<G>(__ballot(f), gw);
+        const int pre = __popcll(bm & below);
+        if (valid && !f) sC[k + r - 2 * (common2 + pre)] = y;
+        common2 += __popcll(bm);
+    }
+    return na + nb - 2 * common;
+}
+
+// Lane-group shapes: G lanes per column, W LDS entries per buffer (three
+// buffers per group), kBlock / G groups per block.
+template <int G, int W>
+struct GroupShape {
+    static constexpr int NG = kBlock / G;
+    static constexpr int LDS_BYTES = NG * 3 * W * 4;
+    static_assert(LDS_BYTES <= 64 * 1024, "static LDS limit");
+};
+
+// Allocate `n` pool entries by the group leader; broadcast to the group.
+// Returns ~0ull when the pool is full.
+template <int G>
+__device__ __forceinline__ unsigned long long group_alloc(const Dev &d, unsigned n, int gl, int gw) {
+    unsigned long long base = 0;
+    if (gl == 0) {
+        base = atomicAdd(&d.ctrl->pool_top, (unsigned long long)n);
+        if (base + n > d.ctrl->pool_cap) base = ~0ull;
+    }
+    return __shfl(base, gw * G, 64);
+}
+
+template <int G, int W>
+__global__ void __launch_bounds__(kBlock) k_phase2_group(Dev d) {
+    constexpr int NG = GroupShape<G, W>::NG;
+    __shared__ int lds[NG * 3 * W];
+    const int gid = threadIdx.x / G;
+    const int gl = threadIdx.x % G;
+    const int gw = (threadIdx.x & 63) / G;
+    const int leader = gw * G;
+    int *const buf = lds + gid * 3 * W;
+    const int nwork = d.ctrl->nwork;
+    const int slot0 = d.slot0;
+    unsigned long long nadd = 0, nel = 0;
+    int spills = 0;
+    bool ovf = false;
+
+    for (int w = blockIdx.x * NG + gid; w < nwork; w += gridDim.x * NG) {
+        const int j = d.work[w];
+        int lj = d.low[j];
+        int4 pd = d.pivdesc[lj];
+        unsigned coff = d.off[j];
+        int clen = d.len[j], ccap = d.cap[j];
+        int *sA = buf, *sB = buf + W, *sC = buf + 2 * W;
+        bool glob = clen > W;  // contents in the pool instead of LDS
+        if (!glob)
+            for (int i = gl; i < clen; i += G) sB[i] = d.pool[coff + i];
+        __syncwarp();
+        bool dirty = false, becomes_pivot = false, lost = false;
+        while (pd.x > -1 && pd.x < j) {
+            const int na = pd.z;
+            const unsigned aoff = (unsigned)pd.y;
+            if (gl == 0) d.ess[lj] = 0;
+            int nc;
+            if (!glob && na + clen <= W) {
+                for (int i = gl; i < na; i += G) sA[i] = d.pool[aoff + i];
+                __syncwarp();
+                nc = group_xor<G>(sA, na, sB, clen, sC, gl, gw);
+                __syncwarp();
+                int *t = sB;
+                sB = sC;
+                sC = t;
+            } else {
+                // The result may not fit in LDS: the leader merges serially
+                // into a fresh pool segment, reading the current contents
+                // from LDS the first time and from the pool afterwards.
+                unsigned long long base = group_alloc<G>(d, (unsigned)(na + clen), gl, gw);
+                if (base == ~0ull) {
+                    ovf = true;
+                    lost = !glob && dirty;  // progress held only in LDS
+                    break;
+                }
+                int n = 0, top = -1;
+                if (gl == 0) {
+                    const int *cur = glob ? d.pool + coff : sB;
+                    n = merge_xor(d.pool + aoff, na, cur, clen, d.pool + base);
+                    top = n ? d.pool[base + n - 1] : -1;
+                }
+                if (!glob) spills++;
+                glob = true;
+                nc = __shfl(n, leader, 64);
+                coff = (unsigned)base;
+                ccap = na + clen;
+                nadd++;
+                nel += (unsigned long long)(na + clen + nc);
+                clen = nc;
+                dirty = true;
+                int nl = __shfl(top, leader, 64);
+                if (nl >= lj) {
+                    if (gl == 0) atomicOr(&d.ctrl->err, kErrState);
+                    break;
+                }
+                lj = nl;
+                if (lj < 0) break;
+                if (gl == 0) d.clear[lj] = 1;
+                if (d.left[lj] == j) {
+                    becomes_pivot = true;
+                    break;
+                }
+                pd = d.pivdesc[lj];
+                continue;
+            }
+            nadd++;
+            nel += (unsigned long long)(na + clen + nc);
+            clen = nc;
+            dirty = true;
+            const int nl = nc ? sB[nc - 1] : -1;
+            if (nl >= lj) {
+                if (gl == 0) atomicOr(&d.ctrl->err, kErrState);
+                break;
+            }
+            lj = nl;
+            if (lj < 0) break;
+            if (gl == 0) d.clear[lj] = 1;
+            if (d.left[lj] == j) {
+                becomes_pivot = true;
+                break;
+            }
+            pd = d.pivdesc[lj];
+        }
+        if (lost || !dirty) continue;
+        if (!glob) {
+            // write back from LDS: in place when the segment is big enough
+            if (clen > ccap) {
+                unsigned want = (unsigned)clen;
+                if (want < (unsigned)slot0) want = (unsigned)slot0;
+                want = (want + 7u) & ~7u;
+                unsigned long long base = group_alloc<G>(d, want, gl, gw);
+                if (base == ~0ull) {
+                    ovf = true;
+                    continue;  // nothing of this column's progress is kept
+                }
+                coff = (unsigned)base;
+                ccap = (int)want;
+            }
+            for (int i = gl; i < clen; i += G) d.pool[coff + i] = sB[i];
+        }
+        if (gl == 0) {
+            d.off[j] = coff;
+            d.len[j] = clen;
+            d.cap[j] = ccap;
+            d.low[j] = lj;
+            if (becomes_pivot) {
+                d.newpiv[lj] = j;
+                d.classes[j] = -1;
+            }
+        }
+    }
+    if (ovf && gl == 0) d.ctrl->overflow = 1;
+    // per-group counters are identical on the group's lanes: count leaders only
+    if (gl != 0) {
+        nadd = 0;
+        nel = 0;
+        spills = 0;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        nadd += __shfl_down(nadd, o, 64);
+        nel += __shfl_down(nel, o, 64);
+        spills += __shfl_down(spills, o, 64);
+    }
+    if (lane_id() == 0) {
+        if (nadd) {
+            atomicAdd(&d.ctrl->additions, nadd);
+            atomicAdd(&d.ctrl->merge_elems, nel);
+        }
+        if (spills) atomicAdd(&d.ctrl->spills, spills);
+    }
+}
+
+// The lane-group shapes that are compiled: {lanes, LDS entries per buffer}.
+using Phase2Kernel = void (*)(Dev);
+
+static Phase2Kernel phase2_kernel(int lanes, int wlds) {
+#define OPH_SHAPE(G, W) \
+    if (lanes == G && wlds == W) return k_phase2_group<G, W>;
+    OPH_SHAPE(4, 32)
+    OPH_SHAPE(4, 64)
+    OPH_SHAPE(8, 32)
+    OPH_SHAPE(8, 64)
+    OPH_SHAPE(8, 128)
+    OPH_SHAPE(16, 64)
+    OPH_SHAPE(16, 128)
+    OPH_SHAPE(16, 256)
+    OPH_SHAPE(32, 256)
+    OPH_SHAPE(32, 512)
+    OPH_SHAPE(64, 512)
+    OPH_SHAPE(64, 1024)
+#undef OPH_SHAPE
+    return nullptr;
+}
+
+bool phase2_shape_ok(int lanes, int wlds) {
+    return lanes == 1 || phase2_kernel(lanes, wlds) != nullptr;
+}
+
+int phase2_default_lds(int lanes) {
+    switch (lanes) {
+        case 4: return 64;
+        case 8: return 64;
+        case 16: return 128;
+        case 32: return 512;
+        case 64: return 1024;
+        default: return 0;
+    }
+}
+
+int phase2_blocks_per_cu(int lanes, int wlds) {
+    Phase2Kernel k = phase2_kernel(lanes, wlds);
+    if (!k) return 0;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(k),
+                                                     kBlock, 0) != hipSuccess || n < 1)
+        n = 1;
+    return n;
+}
+
+void launch_phase2(const Dev &d, int lanes, int wlds, int grid, hipStream_t s) {
+    Phase2Kernel k = phase2_kernel(lanes, wlds);
+    if (!k) {
+        int gm = (d.m + kBlock - 1) / kBlock;
+        k_phase2_lane<<<gm, kBlock, 0, s>>>(d);
+        return;
+    }
+    k<<<grid, kBlock, 0, s>>>(d);
 }
 
 // After Phase II: publish new pivots, clear_positives, ess_hat
@@ -500,7 +1011,7 @@ __global__ void k_apply2(Dev d) {
     if (t >= d.m) return;
     int np = d.newpiv[t];
     if (np > -1) {
-        d.arglow[t] = np;
+        publish_pivot(d, t, np);
         d.newpiv[t] = -1;
     }
     if (d.clear[t]) {
@@ -554,80 +1065,32 @@ void launch_final(const Dev &d, int *low1_out, int *ess_out, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Pool compaction: live columns (not cleared) are copied, in column order,
-// to the other pool; cleared columns are dropped (they are never read
-// again: not reduced, never a pivot).
+// Pool compaction: live columns are copied, in column order, to the other
+// pool; cleared columns are dropped (never reduced again, never a pivot).
 // ---------------------------------------------------------------------------
-
-__device__ __forceinline__ unsigned live_len(const Dev &d, int j) {
-    return (j < d.m && d.classes[j] != 1) ? (unsigned)d.len[j] : 0u;
-}
-
-__global__ void __launch_bounds__(kBlock) k_scan_tiles(Dev d) {
-    __shared__ unsigned long long lds[kBlock / 64];
-    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
-    unsigned long long sum = 0;
-#pragma unroll
-    for (int k = 0; k < kItems; k++) sum += live_len(d, base + k);
-    unsigned long long tot;
-    block_exclusive_scan(sum, SumOp<unsigned long long>(), 0ull, lds, &tot);
-    if (threadIdx.x == 0) d.scan_tmp[blockIdx.x] = tot;
-}
-
-__global__ void __launch_bounds__(1024) k_scan_partials(Dev d) {
-    __shared__ unsigned long long lds[16];
-    unsigned long long running = 0;
-    for (int c0 = 0; c0 < d.ntiles; c0 += blockDim.x) {
-        int t = c0 + threadIdx.x;
-        unsigned long long v = t < d.ntiles ? d.scan_tmp[t] : 0ull;
-        unsigned long long tot;
-        unsigned long long ex = block_exclusive_scan(v, SumOp<unsigned long long>(), 0ull, lds, &tot);
-        __syncthreads();
-        if (t < d.ntiles) d.scan_tmp[t] = ex + running;
-        running += tot;
-    }
-    if (threadIdx.x == 0) {
-        d.scan_tmp[d.ntiles] = running;
-        d.ctrl->pool_top = running;
-    }
-}
-
-__global__ void __launch_bounds__(kBlock) k_scan_apply(Dev d) {
-    __shared__ unsigned long long lds[kBlock / 64];
-    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
-    unsigned v[kItems];
-    unsigned long long sum = 0;
-#pragma unroll
-    for (int k = 0; k < kItems; k++) {
-        v[k] = live_len(d, base + k);
-        sum += v[k];
-    }
-    unsigned long long tot;
-    unsigned long long ex = block_exclusive_scan(sum, SumOp<unsigned long long>(), 0ull, lds, &tot);
-    ex += d.scan_tmp[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < kItems; k++) {
-        int j = base + k;
-        if (j < d.m) d.off2[j] = (unsigned)ex;
-        ex += v[k];
-    }
-}
 
 __global__ void k_copy_cols(Dev d) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= d.m) return;
-    unsigned n = live_len(d, j);
+    unsigned sz = col_size(d, j, 1);
+    int n = d.len[j];
+    if (sz == 0) {
+        d.len[j] = 0;
+        d.cap[j] = 0;
+        return;
+    }
     const int *src = d.pool + d.off[j];
-    int *dst = d.pool2 + d.off2[j];
-    for (unsigned i = 0; i < n; i++) dst[i] = src[i];
-    if (n == 0) d.len[j] = 0;
+    const unsigned no = d.off2[j];
+    int *dst = d.pool2 + no;
+    for (int i = 0; i < n; i++) dst[i] = src[i];
+    d.cap[j] = (int)sz;
+    if (d.classes[j] == -1) {
+        int l = d.low[j];
+        if (l > -1 && d.arglow[l] == j) d.pivdesc[l] = make_int4(j, (int)no, n, 0);
+    }
 }
 
-void launch_compact_scan(const Dev &d, hipStream_t s) {
-    k_scan_tiles<<<d.ntiles, kBlock, 0, s>>>(d);
-    k_scan_partials<<<1, 1024, 0, s>>>(d);
-    k_scan_apply<<<d.ntiles, kBlock, 0, s>>>(d);
-}
+void launch_compact_scan(const Dev &d, hipStream_t s) { launch_scan(d, 1, d.off2, s); }
 
 void launch_compact_copy(Dev &d, hipStream_t s) {
     int gm = (d.m + kBlock - 1) / kBlock;
@@ -638,21 +1101,6 @@ void launch_compact_copy(Dev &d, hipStream_t s) {
     int *tp = d.pool;
     d.pool = d.pool2;
     d.pool2 = tp;
-}
-
-void launch_ctrl_init(const Dev &d, unsigned long long cap, hipStream_t s) {
-    k_ctrl_init<<<1, 64, 0, s>>>(d.ctrl, cap);
-}
-
-__global__ void k_reset_flags(Ctrl *c) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        c->any_dup = 0;
-        c->overflow = 0;
-    }
-}
-
-void launch_reset_flags(const Dev &d, hipStream_t s) {
-    k_reset_flags<<<1, 64, 0, s>>>(d.ctrl);
 }
 
 }  // namespace oph

@@ -70,7 +70,8 @@ struct Context {
     // per-column arrays share cap_m; remember each allocation separately
     size_t c_low = 0, c_arglow = 0, c_classes = 0, c_ess = 0, c_left = 0, c_beta = 0,
            c_first = 0, c_clear = 0, c_newpiv = 0, c_slot = 0, c_len = 0, c_off = 0,
-           c_off2 = 0, c_tmax = 0, c_tpre = 0, c_ctrl = 0;
+           c_off2 = 0, c_tmax = 0, c_tpre = 0, c_ctrl = 0, c_cap = 0, c_work = 0, c_pivdesc = 0;
+    int num_cu = 0;
     // staging for host-pointer entry points
     int *h_rows = nullptr, *h_cols = nullptr, *h_low = nullptr, *h_ess = nullptr;
     size_t c_hrows = 0, c_hcols = 0, c_hlow = 0, c_hess = 0;
@@ -110,6 +111,9 @@ static int ensure(Context &c, int m, int nnz, size_t pool_elems) {
     OPH_CHECK(grow(&d.len, c.c_len, mm));
     OPH_CHECK(grow(&d.off, c.c_off, mm));
     OPH_CHECK(grow(&d.off2, c.c_off2, mm));
+    OPH_CHECK(grow(&d.cap, c.c_cap, mm));
+    OPH_CHECK(grow(&d.work, c.c_work, mm));
+    OPH_CHECK(grow(&d.pivdesc, c.c_pivdesc, mm));
     OPH_CHECK(grow(&d.slotmap, c.cap_slotmap, mm + 1));
     size_t nt = (size_t)(m + kTile - 1) / kTile + 1;
     OPH_CHECK(grow(&d.tile_max, c.c_tmax, nt * kMaxSlots));
@@ -123,26 +127,34 @@ static int ensure(Context &c, int m, int nnz, size_t pool_elems) {
 }
 
 struct RunCfg {
+    int slot0 = 32;
     int max_iters = 1000000;
     size_t pool_elems = 0;
     size_t pool_max = 0xFFFFFFFFull;
     int lanes = 0;
+    int wlds = 0;
     bool track = false;
     bool timing = true;
 };
 
 static RunCfg cfg_from(const openph_options *opt, int nnz) {
     RunCfg r;
-    size_t def = std::max<size_t>((size_t)nnz * 16, (size_t)1 << 26);
-    r.pool_elems = def;
+    (void)nnz;
+    r.pool_elems = 0;  // automatic: the initial layout plus headroom
     if (opt) {
         if (opt->max_iters > 0) r.max_iters = opt->max_iters;
         if (opt->pool_max_elems > 0) r.pool_max = std::min<size_t>((size_t)opt->pool_max_elems, 0xFFFFFFFFull);
         if (opt->pool_elems > 0) r.pool_elems = (size_t)opt->pool_elems;
         r.lanes = opt->lanes_per_column;
+        r.wlds = opt->group_lds;
         r.track = opt->track != 0;
+        if (opt->slot_elems > 0) r.slot0 = std::min(opt->slot_elems, 1 << 16);
     }
     if (r.pool_elems > r.pool_max) r.pool_elems = r.pool_max;
+    if (r.lanes != 0) {
+        if (r.lanes != 1 && !r.wlds) r.wlds = phase2_default_lds(r.lanes);
+        if (!phase2_shape_ok(r.lanes, r.wlds)) r.lanes = -1;  // rejected by run_pms
+    }
     return r;
 }
 
@@ -190,19 +202,33 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
         }
         return OPENPH_OK;
     }
-    int rc = ensure(c, m, nnz, cfg.pool_elems);
+    if (cfg.lanes < 0) return OPENPH_EINVAL;
+    // the initial layout gives every column a segment of >= slot0 entries
+    const size_t layout = (size_t)nnz + (size_t)m * (size_t)cfg.slot0;
+    if (layout > cfg.pool_max) return OPENPH_ENOMEM;
+    size_t pool_want = cfg.pool_elems
+                           ? std::max(cfg.pool_elems, layout)
+                           : layout + std::max<size_t>((size_t)nnz * 8, (size_t)1 << 25);
+    pool_want = std::min(pool_want, cfg.pool_max);
+    int rc = ensure(c, m, nnz, pool_want);
     if (rc) return rc;
     Dev &d = c.d;
     d.m = m;
     d.nnz = nnz;
     d.ntiles = (m + kTile - 1) / kTile;
+    d.slot0 = cfg.slot0;
+    if (!c.num_cu) {
+        hipDeviceProp_t prop;
+        OPH_CHECK(hipGetDeviceProperties(&prop, c.device));
+        c.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    }
     if (!c.ev0) {
         OPH_CHECK(hipEventCreate(&c.ev0));
         OPH_CHECK(hipEventCreate(&c.ev1));
     }
     // logical pool size: the requested one, even when the context holds a
     // larger buffer from an earlier call
-    size_t lcap = std::min(c.cap_pool, std::max(cfg.pool_elems, (size_t)nnz));
+    size_t lcap = std::min(c.cap_pool, pool_want);
     launch_ctrl_init(d, (unsigned long long)lcap, s);
     launch_setup(d, rows1, cols1, s);
     Ctrl h{};
@@ -210,9 +236,15 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
     OPH_CHECK(hipStreamSynchronize(s));
     if ((rc = err_from_bits(h.err))) return rc;
     launch_slots(d, s);
-    // pool_top after the input copy
-    unsigned long long top = (unsigned long long)nnz;
-    OPH_CHECK(hipMemcpyAsync(&d.ctrl->pool_top, &top, sizeof(top), hipMemcpyHostToDevice, s));
+    // Phase II lanes per column: short input columns (simplicial complexes
+    // of small dimension) take 8 lanes with 128-entry LDS buffers, longer
+    // ones (cubical complexes) a full wave with 1024-entry buffers.
+    int lanes = cfg.lanes, wlds = cfg.wlds;
+    if (!lanes) {
+        lanes = h.maxlen <= 4 ? 8 : 64;
+        wlds = h.maxlen <= 4 ? 128 : 1024;
+    }
+    const int grid = lanes > 1 ? phase2_blocks_per_cu(lanes, wlds) * c.num_cu : 0;
     OPH_CHECK(hipEventRecord(c.ev0, s));
     launch_phase0(d, s);
 
@@ -259,10 +291,10 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
                 c.p2_events.push_back(e);
             }
             OPH_CHECK(hipEventRecord(c.p2_events[k], s));
-            launch_phase2(d, cfg.lanes, s);
+            launch_phase2(d, lanes, wlds, grid, s);
             OPH_CHECK(hipEventRecord(c.p2_events[k + 1], s));
         } else {
-            launch_phase2(d, cfg.lanes, s);
+            launch_phase2(d, lanes, wlds, grid, s);
         }
         launch_apply2(d, s);
         if (time_track && (size_t)iter <= (size_t)m) {
@@ -281,7 +313,7 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
         // occurrence of the largest repeated low is marked, or its repeats
         // are reduced); no progress without an overflow means a fault.
         bool progress = h.additions != prev_add || h.marks != prev_marks;
-        if (!h.overflow && !progress) return OPENPH_EITER;
+        if (!h.overflow && !progress && h.any_dup) return OPENPH_EITER;
         // a full pool with no progress twice in a row (the second time right
         // after a compaction) means the pool budget cannot hold the work
         stalls = (h.overflow && !progress) ? stalls + 1 : 0;
@@ -311,6 +343,9 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
         OPH_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
         stats->device_ms = ms;
         stats->m = m;
+        stats->lanes = lanes;
+        stats->group_lds = wlds;
+        stats->spills = h.spills;
         stats->phase2_launches = iter - 1;
         float p2 = 0.f;
         for (int i = 0; i < iter - 1; i++) {

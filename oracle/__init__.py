@@ -138,12 +138,12 @@ def pms(b, max_iters: int = 0, iter_stats: bool = False, snapshot: bool = False)
     st = np.zeros(4, np.int64)
     nit = np.zeros(1, np.int32)
     cap = 100000
-    its = np.zeros(4 * cap, np.int64) if iter_stats else None
+    its = np.zeros(5 * cap, np.int64) if iter_stats else None
     ptr = its.ctypes.data if its is not None else None
     _check(_lib().orc_pms(m, colptr, rows, low, ess, nit, st, ptr, cap, max_iters, int(snapshot)), "pms")
     per = None
     if its is not None:
-        per = its.reshape(-1, 4)[: int(st[3])].copy()
+        per = its.reshape(-1, 5)[: int(st[3])].copy()
     return OracleResult(low, ess, st, int(nit[0]), per)
 
 

@@ -302,9 +302,9 @@ static void left_beta(int m, const int64_t *colptr, const int32_t *rows,
         }
 }
 
-/* iter_stats (optional): per iteration 4 int64 -
+/* iter_stats (optional): per iteration 5 int64 -
  *   additions, merge elements, columns marked in Phase I, columns that
- *   entered Phase II's loop. */
+ *   entered Phase II's loop, longest chain of additions of one column. */
 int orc_pms(int m, const int64_t *colptr, const int32_t *rows, int32_t *low,
             int32_t *ess, int32_t *num_iters, int64_t *stats, int64_t *iter_stats,
             int max_iter_stats, int max_iters, int snapshot) {
@@ -349,7 +349,7 @@ int orc_pms(int m, const int64_t *colptr, const int32_t *rows, int32_t *low,
     int converged = 0;
     while (!converged) {
         if (max_iters > 0 && iter > max_iters) { rc = ORC_EOVERFLOW; break; }
-        int64_t add0 = st[ST_ADD], el0 = st[ST_ELEMS], marked = 0, entered = 0;
+        int64_t add0 = st[ST_ADD], el0 = st[ST_ELEMS], marked = 0, entered = 0, maxchain = 0;
         /* Phase I: transverse_dimensions (pms_functions.cu:55-85) */
         for (int j = 0; j < m; j++) { visited[j] = 0; clear[j] = 0; }
         for (int d = 0; d < ndim; d++) ceil_d[d] = -1;
@@ -383,6 +383,7 @@ int orc_pms(int m, const int64_t *colptr, const int32_t *rows, int32_t *low,
             int lj = low[j];
             int pivot = lj > -1 ? arglow[lj] : -1;
             if (-1 < pivot && pivot < j && classes[j] == 0) entered++;
+            int64_t chain0 = st[ST_ADD];
             while (-1 < pivot && pivot < j && classes[j] == 0) {
                 if (low[j] > -1) ess[low[j]] = 0;
                 lj = dyn_add(&s, pivot, j, st, &rc);
@@ -399,6 +400,7 @@ int orc_pms(int m, const int64_t *colptr, const int32_t *rows, int32_t *low,
                     pivot = -1;
                 }
             }
+            if (st[ST_ADD] - chain0 > maxchain) maxchain = st[ST_ADD] - chain0;
         }
         if (rc) break;
         for (int t = 0; t < m; t++) {
@@ -433,11 +435,12 @@ int orc_pms(int m, const int64_t *colptr, const int32_t *rows, int32_t *low,
             }
         }
         if (iter_stats && iter - 1 < max_iter_stats) {
-            int64_t *q = iter_stats + 4 * (int64_t)(iter - 1);
+            int64_t *q = iter_stats + 5 * (int64_t)(iter - 1);
             q[0] = st[ST_ADD] - add0;
             q[1] = st[ST_ELEMS] - el0;
             q[2] = marked;
             q[3] = entered;
+            q[4] = maxchain;
         }
         iter++;
     }

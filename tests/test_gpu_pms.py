@@ -53,6 +53,47 @@ def test_pms_matches_golden(name):
     assert st.m == b.m
 
 
+SHAPES = [(1, 0), (4, 32), (4, 64), (8, 32), (8, 64), (8, 128), (16, 64), (16, 128),
+          (16, 256), (32, 256), (32, 512), (64, 512), (64, 1024)]
+
+
+@pytest.mark.parametrize("lanes,wlds", SHAPES)
+@pytest.mark.parametrize("name", ["torus100", "cubical16", "vr20_sphere"])
+def test_every_lane_configuration_is_exact(name, lanes, wlds):
+    """The Phase II schedule does not depend on the lane configuration, so
+    the counters must match the oracle's snapshot schedule as well."""
+    say("lanes", name, lanes, wlds)
+    f = np.load(os.path.join(GOLD, name + ".npz"))
+    opt = openph_amd.Options()
+    opt.lanes_per_column = lanes
+    opt.group_lds = wlds
+    low, ess, st = openph_amd.reduce(_gen(name), "pms", opt)
+    assert st.lanes == lanes
+    assert np.array_equal(low, f["low"])
+    assert np.array_equal(ess, f["ess"])
+    snap = INDEX[name]["pms_snapshot"]
+    assert st.overflows == 0
+    assert st.additions == snap["additions"]
+    assert st.merge_elems == snap["merge_elems"]
+    assert st.num_iters == snap["num_iters"]
+
+
+@pytest.mark.parametrize("lanes", [4, 8])
+def test_lds_overflow_path_is_exact(lanes):
+    """Cubical columns outgrow a lane group's LDS buffers (16 x lanes entries);
+    the global-memory fallback must give the same result."""
+    say("lds overflow", lanes)
+    b = C.cubical64(seed=40, side=40)
+    ref = oracle.pms(b, snapshot=True)
+    opt = openph_amd.Options()
+    opt.lanes_per_column = lanes
+    low, ess, st = openph_amd.reduce(b, "pms", opt)
+    assert st.spills > 0
+    assert np.array_equal(low, ref.low)
+    assert np.array_equal(ess, ref.ess)
+    assert st.additions == ref.additions
+
+
 @pytest.mark.parametrize("algo", openph_amd.ALGORITHMS)
 def test_every_algstr_gives_the_same_low(algo):
     say("algo", algo)
