@@ -75,12 +75,75 @@ def cpu_baseline(b, budget_s: float = 10.0):
     }, low
 
 
+def batched_leg(count: int = 512, reps: int = 3):
+    """512 VR 2-skeleta (N ~ U{20..80}, mixed shapes) reduced (a) by one
+    batched call and (b) by `count` sequential plain calls, inputs resident in
+    HBM for both; outputs compared bit for bit."""
+    import torch
+
+    import openph_amd
+    from openph_amd import complexes as C
+
+    t0 = time.perf_counter()
+    mats = C.batched_mix(count, seed=0)
+    gen_s = time.perf_counter() - t0
+    ms = np.array([b.m for b in mats], np.int32)
+    nnzs = np.array([b.nnz for b in mats], np.int32)
+    rr, cc = zip(*(b.coo1() for b in mats))
+    dev = torch.device("cuda:0")
+    rows = torch.from_numpy(np.concatenate(rr)).to(dev)
+    cols = torch.from_numpy(np.concatenate(cc)).to(dev)
+    M = int(ms.sum())
+    low_b = torch.empty(M, dtype=torch.int32, device=dev)
+    ess_b = torch.empty(M, dtype=torch.int32, device=dev)
+    low_p = torch.empty(M, dtype=torch.int32, device=dev)
+    ess_p = torch.empty(M, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    moff = np.concatenate([[0], np.cumsum(ms)]).astype(np.int64)
+    noff = np.concatenate([[0], np.cumsum(nnzs)]).astype(np.int64)
+
+    def plain_pass():
+        for k in range(count):
+            a, b_ = int(noff[k]), int(noff[k + 1])
+            x, y = int(moff[k]), int(moff[k + 1])
+            openph_amd.pms_device(rows[a:b_], cols[a:b_], y - x, low_p[x:y], ess_p[x:y],
+                                  stream=stream)
+
+    # warm both paths once
+    st_b = openph_amd.pms_batch_device(ms, nnzs, rows, cols, low_b, ess_b, stream=stream)
+    plain_pass()
+    torch.cuda.synchronize()
+    tb = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        st_b = openph_amd.pms_batch_device(ms, nnzs, rows, cols, low_b, ess_b, stream=stream)
+        torch.cuda.synchronize()
+        tb.append(time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    plain_pass()
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    same = bool(torch.equal(low_b, low_p) and torch.equal(ess_b, ess_p))
+    b_s = statistics.median(tb)
+    return {
+        "workload": f"{count} VR 2-skeleta, N~U{{20..80}}, cube/sphere/5-GMM, seed 0",
+        "columns": M, "nnz": int(nnzs.sum()),
+        "batched_cols_per_s": M / b_s, "batched_ms": b_s * 1e3,
+        "plain_sequential_cols_per_s": M / tp, "plain_sequential_ms": tp * 1e3,
+        "speedup_batched_vs_plain": tp / b_s,
+        "batched_num_iters": st_b.num_iters, "batched_additions": st_b.additions,
+        "batched_merge_elems": st_b.merge_elems,
+        "batched_equals_plain": same, "host_generation_s": gen_s,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus %d is not "
@@ -146,7 +209,8 @@ def main():
                    "algo": "pms", "inputs": "1-based COO resident in HBM"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": "k_phase2_lane",
+                     "kernel": (f"k_phase2_group<{st.lanes}, {st.group_lds}>" if st.lanes > 1
+                                else "k_phase2_lane"),
                      "bytes_per_launch": p2_bytes / max(1, launches),
                      "avg_launch_ms": p2_ms / max(1, launches)},
         "hbm_alg": {"bytes": b_alg, "device_ms": dev_ms,
@@ -156,7 +220,11 @@ def main():
                 "pool_peak": st.pool_peak, "additions_per_s": st.additions * args.steps / wall,
                 "parity_vs_golden": ok},
     }
+    if not args.no_batched:
+        print("bench: batched leg", file=sys.stderr, flush=True)
+        out["batched"] = batched_leg()
     if not args.no_cpu:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
         cb, cpu_low = cpu_baseline(b)
         cb["low_sha_match"] = bool(np.array_equal(cpu_low, gold["low"]))
         out["cpu_baseline"] = cb

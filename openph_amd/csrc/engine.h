@@ -73,6 +73,12 @@ void launch_apply1(const Dev &d, hipStream_t s);
 // the worklist with wlds LDS entries per buffer. grid = blocks for the group
 // kernels.
 void launch_phase2(const Dev &d, int lanes, int wlds, int grid, hipStream_t s);
+// Batches: local 1-based COO of `count` matrices -> global block-diagonal
+// COO (err gets bit 1 for an index outside its matrix), and global 1-based
+// low -> local 1-based low.
+void launch_batch_offsets(const int *rows1, const int *cols1, const int *noff, const int *moff,
+                          int count, int nnz, int *rows_g, int *cols_g, int *err, hipStream_t s);
+void launch_batch_local_low(int *low1, const int *moff, int count, int m, hipStream_t s);
 bool phase2_shape_ok(int lanes, int wlds);
 int phase2_default_lds(int lanes);
 int phase2_blocks_per_cu(int lanes, int wlds);

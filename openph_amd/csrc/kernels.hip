@@ -1103,4 +1103,56 @@ void launch_compact_copy(Dev &d, hipStream_t s) {
     d.pool2 = tp;
 }
 
+// ---------------------------------------------------------------------------
+// Batches: `count` matrices reduced as one block-diagonal matrix. Matrix k
+// owns columns and rows [moff[k], moff[k+1]) and COO entries
+// [noff[k], noff[k+1]); its input indices are local and 1-based.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int segment_of(const int *off, int count, int x) {
+    // largest k with off[k] <= x (off has count+1 ascending entries)
+    int lo = 0, hi = count;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (off[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_batch_offsets(const int *__restrict__ rows1, const int *__restrict__ cols1,
+                                const int *__restrict__ noff, const int *__restrict__ moff,
+                                int count, int nnz, int *__restrict__ rows_g,
+                                int *__restrict__ cols_g, int *err) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    const int k = segment_of(noff, count, e);
+    const int base = moff[k], mk = moff[k + 1] - base;
+    const int r = rows1[e], c = cols1[e];
+    if (r < 1 || c < 1 || r > mk || c > mk) atomicOr(err, 1);
+    rows_g[e] = r + base;
+    cols_g[e] = c + base;
+}
+
+__global__ void k_batch_local_low(int *__restrict__ low1, const int *__restrict__ moff, int count,
+                                  int m) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const int l = low1[j];
+    if (l > 0) low1[j] = l - moff[segment_of(moff, count, j)];
+}
+
+void launch_batch_offsets(const int *rows1, const int *cols1, const int *noff, const int *moff,
+                          int count, int nnz, int *rows_g, int *cols_g, int *err, hipStream_t s) {
+    if (nnz > 0)
+        k_batch_offsets<<<(nnz + kBlock - 1) / kBlock, kBlock, 0, s>>>(rows1, cols1, noff, moff,
+                                                                      count, nnz, rows_g, cols_g,
+                                                                      err);
+}
+
+void launch_batch_local_low(int *low1, const int *moff, int count, int m, hipStream_t s) {
+    if (m > 0)
+        k_batch_local_low<<<(m + kBlock - 1) / kBlock, kBlock, 0, s>>>(low1, moff, count, m);
+}
+
 }  // namespace oph

@@ -75,6 +75,10 @@ struct Context {
     // staging for host-pointer entry points
     int *h_rows = nullptr, *h_cols = nullptr, *h_low = nullptr, *h_ess = nullptr;
     size_t c_hrows = 0, c_hcols = 0, c_hlow = 0, c_hess = 0;
+    // batches: per-matrix offsets and the block-diagonal COO
+    int *b_moff = nullptr, *b_noff = nullptr, *b_rows = nullptr, *b_cols = nullptr;
+    int *b_err = nullptr;
+    size_t c_bmoff = 0, c_bnoff = 0, c_brows = 0, c_bcols = 0, c_berr = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> iter_events;
@@ -418,6 +422,91 @@ static int run_host(const char *algstr, const int *rows1, const int *cols1, int 
     return OPENPH_OK;
 }
 
+// Batch of `count` matrices (device COO, local 1-based indices) reduced as
+// one block-diagonal matrix. low1/ess (device) receive the concatenated
+// per-matrix outputs with local 1-based lows.
+static int run_batch(Context &c, int count, const int *ms, const int *nnzs, const int *rows1,
+                     const int *cols1, int *low1, int *ess, hipStream_t s, const RunCfg &cfg,
+                     openph_stats *stats) {
+    if (count < 0 || (count > 0 && (!ms || !nnzs))) return OPENPH_EINVAL;
+    std::vector<int> moff((size_t)count + 1), noff((size_t)count + 1);
+    long long M = 0, N = 0;
+    for (int k = 0; k < count; k++) {
+        if (ms[k] < 0 || nnzs[k] < 0) return OPENPH_EINVAL;
+        moff[k] = (int)M;
+        noff[k] = (int)N;
+        M += ms[k];
+        N += nnzs[k];
+        if (M > 0x7fffffffLL || N > 0x7fffffffLL) return OPENPH_EINVAL;
+    }
+    moff[count] = (int)M;
+    noff[count] = (int)N;
+    if (N > 0 && (!rows1 || !cols1)) return OPENPH_EINVAL;
+    if (M > 0 && (!low1 || !ess)) return OPENPH_EINVAL;
+    OPH_CHECK(grow(&c.b_moff, c.c_bmoff, (size_t)count + 1));
+    OPH_CHECK(grow(&c.b_noff, c.c_bnoff, (size_t)count + 1));
+    OPH_CHECK(grow(&c.b_rows, c.c_brows, (size_t)N + 1));
+    OPH_CHECK(grow(&c.b_cols, c.c_bcols, (size_t)N + 1));
+    OPH_CHECK(grow(&c.b_err, c.c_berr, 1));
+    int zero = 0, err = 0;
+    OPH_CHECK(hipMemcpyAsync(c.b_moff, moff.data(), sizeof(int) * (count + 1),
+                             hipMemcpyHostToDevice, s));
+    OPH_CHECK(hipMemcpyAsync(c.b_noff, noff.data(), sizeof(int) * (count + 1),
+                             hipMemcpyHostToDevice, s));
+    OPH_CHECK(hipMemcpyAsync(c.b_err, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+    launch_batch_offsets(rows1, cols1, c.b_noff, c.b_moff, count, (int)N, c.b_rows, c.b_cols,
+                         c.b_err, s);
+    OPH_CHECK(hipMemcpyAsync(&err, c.b_err, sizeof(int), hipMemcpyDeviceToHost, s));
+    OPH_CHECK(hipStreamSynchronize(s));
+    if (err) return OPENPH_EINVAL;
+    int rc = run_pms(c, c.b_rows, c.b_cols, (int)M, (int)N, low1, ess, s, cfg, stats, nullptr);
+    if (rc) return rc;
+    launch_batch_local_low(low1, c.b_moff, count, (int)M, s);
+    OPH_CHECK(hipStreamSynchronize(s));
+    return OPENPH_OK;
+}
+
+static int batch_host(int count, const int *ms, const int *nnzs, const int *rows1,
+                      const int *cols1, const openph_options *opt, int *low1, int *ess,
+                      openph_stats *stats) {
+    if (count < 0 || (count > 0 && (!ms || !nnzs))) return OPENPH_EINVAL;
+    long long M = 0, N = 0;
+    for (int k = 0; k < count; k++) {
+        if (ms[k] < 0 || nnzs[k] < 0) return OPENPH_EINVAL;
+        M += ms[k];
+        N += nnzs[k];
+    }
+    if (M > 0x7fffffffLL || N > 0x7fffffffLL) return OPENPH_EINVAL;
+    if ((N > 0 && (!rows1 || !cols1)) || (M > 0 && (!low1 || !ess))) return OPENPH_EINVAL;
+    int dev = opt ? opt->device : 0;
+    int rc = set_device(dev);
+    if (rc) return rc;
+    Context *c = context_for(dev);
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!c->stream) OPH_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    hipStream_t s = c->stream;
+    OPH_CHECK(grow(&c->h_rows, c->c_hrows, (size_t)N + 1));
+    OPH_CHECK(grow(&c->h_cols, c->c_hcols, (size_t)N + 1));
+    OPH_CHECK(grow(&c->h_low, c->c_hlow, (size_t)M + 1));
+    OPH_CHECK(grow(&c->h_ess, c->c_hess, (size_t)M + 1));
+    if (N) {
+        OPH_CHECK(hipMemcpyAsync(c->h_rows, rows1, sizeof(int) * N, hipMemcpyHostToDevice, s));
+        OPH_CHECK(hipMemcpyAsync(c->h_cols, cols1, sizeof(int) * N, hipMemcpyHostToDevice, s));
+    }
+    RunCfg cfg = cfg_from(opt, (int)N);
+    rc = run_batch(*c, count, ms, nnzs, c->h_rows, c->h_cols, c->h_low, c->h_ess, s, cfg, stats);
+    if (rc) {
+        (void)hipStreamSynchronize(s);
+        return rc;
+    }
+    if (M) {
+        OPH_CHECK(hipMemcpyAsync(low1, c->h_low, sizeof(int) * M, hipMemcpyDeviceToHost, s));
+        OPH_CHECK(hipMemcpyAsync(ess, c->h_ess, sizeof(int) * M, hipMemcpyDeviceToHost, s));
+    }
+    OPH_CHECK(hipStreamSynchronize(s));
+    return OPENPH_OK;
+}
+
 }  // namespace oph
 
 using namespace oph;
@@ -482,17 +571,22 @@ void openph(char *algstr, int *h_rows_in, int *h_cols_in, int m, int col_width, 
 int openph_pms_batch(int count, const int *ms, const int *nnzs, const int *rows1,
                      const int *cols1, const openph_options *opt, int *low1, int *ess,
                      openph_stats *stats) {
-    (void)count; (void)ms; (void)nnzs; (void)rows1; (void)cols1; (void)opt; (void)low1;
-    (void)ess; (void)stats;
-    return OPENPH_EINVAL;
+    return batch_host(count, ms, nnzs, rows1, cols1, opt, low1, ess, stats);
 }
 
 int openph_pms_batch_device(int count, const int *ms, const int *nnzs, const int *rows1,
                             const int *cols1, int *low1, int *ess, void *stream,
                             const openph_options *opt, openph_stats *stats) {
-    (void)count; (void)ms; (void)nnzs; (void)rows1; (void)cols1; (void)low1; (void)ess;
-    (void)stream; (void)opt; (void)stats;
-    return OPENPH_EINVAL;
+    int dev = opt ? opt->device : 0;
+    int rc = set_device(dev);
+    if (rc) return rc;
+    Context *c = context_for(dev);
+    std::lock_guard<std::mutex> g(c->mu);
+    long long N = 0;
+    for (int k = 0; k < count && nnzs; k++) N += nnzs[k] > 0 ? nnzs[k] : 0;
+    RunCfg cfg = cfg_from(opt, (int)std::min<long long>(N, 0x7fffffffLL));
+    return run_batch(*c, count, ms, nnzs, rows1, cols1, low1, ess, (hipStream_t)stream, cfg,
+                     stats);
 }
 
 }  // extern "C"

@@ -155,7 +155,8 @@ def test_small_pool_compacts_and_stays_exact():
     b = C.torus100()
     f = np.load(os.path.join(GOLD, "torus100.npz"))
     opt = openph_amd.Options()
-    opt.pool_elems = b.nnz + 4096
+    opt.slot_elems = 1
+    opt.pool_elems = b.nnz + b.m + 4096
     low, ess, st = openph_amd.reduce(b, "pms", opt)
     assert st.compactions > 0 and st.overflows > 0
     assert np.array_equal(low, f["low"])
@@ -239,3 +240,83 @@ def test_dropin_openph_matches_openph_m_fields():
               "num_iters", "time_inner", "m"):
         assert k in out
     assert t > 0
+
+
+# ---------------------------------------------------------------------------
+# batches (block-diagonal offsets)
+# ---------------------------------------------------------------------------
+
+def test_batch_of_golden_matrices_matches_each_alone():
+    say("batch golden")
+    names = ["vr12", "random0", "random1", "random2", "random3", "vr20_sphere", "cubical8",
+             "vr40", "torus100"]
+    mats = [_gen(n) for n in names]
+    out, st = openph_amd.pms_batch(mats)
+    for n, (low, ess) in zip(names, out):
+        f = np.load(os.path.join(GOLD, n + ".npz"))
+        assert np.array_equal(low, f["low"]), n
+        assert np.array_equal(ess, f["ess"]), n
+    # matrices do not interact: the work is the sum of the parts
+    assert st.additions == sum(INDEX[n]["pms_snapshot"]["additions"] for n in names)
+    assert st.merge_elems == sum(INDEX[n]["pms_snapshot"]["merge_elems"] for n in names)
+    assert st.num_iters == max(INDEX[n]["pms_snapshot"]["num_iters"] for n in names)
+
+
+def test_batch_mix_subset_vs_oracle():
+    say("batch mix")
+    mats = C.batched_mix(24, seed=5)
+    out, st = openph_amd.pms_batch(mats)
+    adds = 0
+    for b, (low, ess) in zip(mats, out):
+        ref = oracle.pms(b, snapshot=True)
+        assert np.array_equal(low, ref.low), b.name
+        assert np.array_equal(ess, ref.ess), b.name
+        adds += ref.additions
+    assert st.additions == adds
+
+
+def test_batch_device_with_empty_members():
+    import torch
+    say("batch device")
+    empty = C.Boundary(colptr=np.zeros(1, np.int64), rows=np.zeros(0, np.int32),
+                       dim=np.zeros(0, np.int32), value=np.zeros(0))
+    verts = C.Boundary(colptr=np.zeros(4, np.int64), rows=np.zeros(0, np.int32),
+                       dim=np.zeros(3, np.int32), value=np.zeros(3))
+    mats = [_gen("vr12"), empty, verts, _gen("random2"), empty]
+    rr, cc = zip(*(b.coo1() for b in mats))
+    rows = torch.from_numpy(np.concatenate(rr)).cuda()
+    cols = torch.from_numpy(np.concatenate(cc)).cuda()
+    M = sum(b.m for b in mats)
+    low = torch.full((M,), -9, dtype=torch.int32, device="cuda")
+    ess = torch.full((M,), -9, dtype=torch.int32, device="cuda")
+    openph_amd.pms_batch_device([b.m for b in mats], [b.nnz for b in mats], rows, cols, low, ess)
+    torch.cuda.synchronize()
+    low, ess = low.cpu().numpy(), ess.cpu().numpy()
+    o = 0
+    for b in mats:
+        ref = oracle.pms(b)
+        assert np.array_equal(low[o:o + b.m] - 1, ref.low)
+        assert np.array_equal(ess[o:o + b.m], ref.ess)
+        o += b.m
+
+
+def test_batch_rejects_index_outside_its_matrix():
+    say("batch invalid")
+    a, b = _gen("random0"), _gen("random1")
+    r, c = b.coo1()
+    c = c.copy()
+    c[-1] = b.m + 3  # points into the next matrix's range
+    bad = C.Boundary(colptr=b.colptr, rows=b.rows, dim=b.dim, value=b.value)
+    ms = np.array([b.m, a.m], np.int32)
+    nnzs = np.array([b.nnz, a.nnz], np.int32)
+    ra, ca = a.coo1()
+    rows = np.concatenate([r, ra]).astype(np.int32)
+    cols = np.concatenate([c, ca]).astype(np.int32)
+    low = np.zeros(int(ms.sum()), np.int32)
+    ess = np.zeros(int(ms.sum()), np.int32)
+    import ctypes
+    L = openph_amd.lib()
+    ip = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    rc = L.openph_pms_batch(2, ip(ms), ip(nnzs), ip(rows), ip(cols), None, ip(low), ip(ess), None)
+    assert rc == -1
+    del bad
