@@ -713,68 +713,75 @@ __global__ void __launch_bounds__(kBlock) k_phase2_lane(Dev d) {
     }
 }
 
-// Lane-group merge. G lanes of one wave own one column; its current contents
-// sit in LDS (sB), the pivot column is staged into sA, and the result goes to
-// sC. Rank form of the sorted symmetric difference: an element x of A that is
-// not in B lands at i + #{b < x} - 2 #{common < x}; #{common < x} is the number
-// of A-elements before x that were found in B, a prefix count taken with a
-// ballot per chunk of G elements (and symmetrically for B).
-template <int G>
-__device__ __forceinline__ unsigned long long group_bits(unsigned long long b, int gw) {
-    if constexpr (G == 64) return b;
-    else return (b >> (gw * G)) & ((1ull << G) - 1ull);
-}
+// Lane-group merge. G lanes of one wave own one column; c = a xor b for
+// ascending lists by merge path. Lane g takes the diagonals [g*K, (g+1)*K) of
+// the merge of a and b (K = ceil(n/G), elements of a before equal elements of
+// b), finds its start by one binary search along its first diagonal and then
+// walks K steps reading one element per step. An element of a is dropped when
+// the next element of b equals it; an element of b is dropped when the
+// previous element of a equals it: both copies of a common element vanish
+// wherever the diagonals split them. A first walk counts the survivors, an
+// exclusive scan over the group gives each lane its output offset and a second
+// walk writes them. The pointers may address LDS or global memory.
 
-__device__ __forceinline__ int lds_lower_bound(const int *s, int n, int x) {
-    int lo = 0;
-    while (n > 0) {
-        int half = n >> 1;
-        if (s[lo + half] < x) {
-            lo += half + 1;
-            n -= half + 1;
-        } else {
-            n = half;
-        }
+// number of a-elements among the first d elements of the merge
+__device__ __forceinline__ int mp_split(const int *a, int na, const int *b, int nb, int d) {
+    int lo = d - nb > 0 ? d - nb : 0, hi = d < na ? d : na;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] <= b[d - mid - 1]) lo = mid + 1;
+        else hi = mid;
     }
     return lo;
 }
 
+template <bool WRITE>
+__device__ __forceinline__ int mp_walk(const int *a, int na, const int *b, int nb, int d0, int d1,
+                                       int *c) {
+    int i = mp_split(a, na, b, nb, d0), k = d0 - i;
+    int av = i < na ? a[i] : kIntMax;
+    int bv = k < nb ? b[k] : kIntMax;
+    int prev_a = i > 0 ? a[i - 1] : -1;
+    int cnt = 0;
+    for (int d = d0; d < d1; d++) {
+        int v;
+        bool keep;
+        if (av <= bv) {
+            v = av;
+            keep = bv != av;
+            prev_a = av;
+            i++;
+            av = i < na ? a[i] : kIntMax;
+        } else {
+            v = bv;
+            keep = prev_a != bv;
+            k++;
+            bv = k < nb ? b[k] : kIntMax;
+        }
+        if (keep) {
+            if (WRITE) c[cnt] = v;
+            cnt++;
+        }
+    }
+    return cnt;
+}
+
 template <int G>
-__device__ int group_xor(const int *sA, int na, const int *sB, int nb, int *sC, int gl, int gw) {
-    const unsigned long long below = (1ull << gl) - 1ull;
-    int common = 0;
-    for (int base = 0; base < na; base += G) {
-        const int i = base + gl;
-        const bool valid = i < na;
-        int x = 0, r = 0;
-        bool f = false;
-        if (valid) {
-            x = sA[i];
-            r = lds_lower_bound(sB, nb, x);
-            f = r < nb && sB[r] == x;
-        }
-        const unsigned long long bm = group_bits<G>(__ballot(f), gw);
-        const int pre = __popcll(bm & below);
-        if (valid && !f) sC[i + r - 2 * (common + pre)] = x;
-        common += __popcll(bm);
+__device__ int group_xor(const int *a, int na, const int *b, int nb, int *c, int gl) {
+    const int n = na + nb;
+    const int K = (n + G - 1) / G;
+    const int d0 = gl * K < n ? gl * K : n;
+    const int d1 = d0 + K < n ? d0 + K : n;
+    const int cnt = mp_walk<false>(a, na, b, nb, d0, d1, nullptr);
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+        const int t = __shfl_up(incl, o, G);
+        if (gl >= o) incl += t;
     }
-    int common2 = 0;
-    for (int base = 0; base < nb; base += G) {
-        const int k = base + gl;
-        const bool valid = k < nb;
-        int y = 0, r = 0;
-        bool f = false;
-        if (valid) {
-            y = sB[k];
-            r = lds_lower_bound(sA, na, y);
-            f = r < na && sA[r] == y;
-        }
-        const unsigned long long bm = group_bits<G>(__ballot(f), gw);
-        const int pre = __popcll(bm & below);
-        if (valid && !f) sC[k + r - 2 * (common2 + pre)] = y;
-        common2 += __popcll(bm);
-    }
-    return na + nb - 2 * common;
+    const int total = __shfl(incl, G - 1, G);
+    mp_walk<true>(a, na, b, nb, d0, d1, c + (incl - cnt));
+    return total;
 }
 
 // Lane-group shapes: G lanes per column, W LDS entries per buffer (three
@@ -805,7 +812,6 @@ __global__ void __launch_bounds__(kBlock) k_phase2_group(Dev d) {
     const int gid = threadIdx.x / G;
     const int gl = threadIdx.x % G;
     const int gw = (threadIdx.x & 63) / G;
-    const int leader = gw * G;
     int *const buf = lds + gid * 3 * W;
     const int nwork = d.ctrl->nwork;
     const int slot0 = d.slot0;
@@ -829,60 +835,40 @@ __global__ void __launch_bounds__(kBlock) k_phase2_group(Dev d) {
             const int na = pd.z;
             const unsigned aoff = (unsigned)pd.y;
             if (gl == 0) d.ess[lj] = 0;
-            int nc;
+            int nc, nl;
             if (!glob && na + clen <= W) {
                 for (int i = gl; i < na; i += G) sA[i] = d.pool[aoff + i];
                 __syncwarp();
-                nc = group_xor<G>(sA, na, sB, clen, sC, gl, gw);
+                nc = group_xor<G>(sA, na, sB, clen, sC, gl);
                 __syncwarp();
                 int *t = sB;
                 sB = sC;
                 sC = t;
+                nl = nc ? sB[nc - 1] : -1;
             } else {
-                // The result may not fit in LDS: the leader merges serially
-                // into a fresh pool segment, reading the current contents
-                // from LDS the first time and from the pool afterwards.
+                // The result may not fit in LDS: the group merges straight
+                // into a fresh pool segment, reading the current contents from
+                // LDS the first time and from the pool afterwards.
                 unsigned long long base = group_alloc<G>(d, (unsigned)(na + clen), gl, gw);
                 if (base == ~0ull) {
                     ovf = true;
                     lost = !glob && dirty;  // progress held only in LDS
                     break;
                 }
-                int n = 0, top = -1;
-                if (gl == 0) {
-                    const int *cur = glob ? d.pool + coff : sB;
-                    n = merge_xor(d.pool + aoff, na, cur, clen, d.pool + base);
-                    top = n ? d.pool[base + n - 1] : -1;
-                }
+                const int *cur = glob ? d.pool + coff : sB;
+                nc = group_xor<G>(d.pool + aoff, na, cur, clen, d.pool + base, gl);
+                // the group's stores must be visible to its own later loads
+                __threadfence_block();
                 if (!glob) spills++;
                 glob = true;
-                nc = __shfl(n, leader, 64);
                 coff = (unsigned)base;
                 ccap = na + clen;
-                nadd++;
-                nel += (unsigned long long)(na + clen + nc);
-                clen = nc;
-                dirty = true;
-                int nl = __shfl(top, leader, 64);
-                if (nl >= lj) {
-                    if (gl == 0) atomicOr(&d.ctrl->err, kErrState);
-                    break;
-                }
-                lj = nl;
-                if (lj < 0) break;
-                if (gl == 0) d.clear[lj] = 1;
-                if (d.left[lj] == j) {
-                    becomes_pivot = true;
-                    break;
-                }
-                pd = d.pivdesc[lj];
-                continue;
+                nl = nc ? d.pool[base + nc - 1] : -1;
             }
             nadd++;
             nel += (unsigned long long)(na + clen + nc);
             clen = nc;
             dirty = true;
-            const int nl = nc ? sB[nc - 1] : -1;
             if (nl >= lj) {
                 if (gl == 0) atomicOr(&d.ctrl->err, kErrState);
                 break;
