@@ -137,6 +137,99 @@ def batched_leg(count: int = 512, reps: int = 3):
     }
 
 
+def served_leg(n_pool: int = 24, n_requests: int = 72, workers: int = 4,
+               loads=(1.0, 8.0), seed: int = 0):
+    """Served pms on the configs[4] mix: 70% VR 2-skeleta of N ~ U{30..200}
+    uniform points, 30% cubical lower-star grids of side ~ U{16..96}.
+
+    A pool of `n_pool` requests is generated once (seed 0); the stream cycles
+    through it. Plain: the requests one after another through the drop-in
+    (openph_v2, host COO in, host low/ess out). Served: Poisson arrivals at a
+    fixed QPS = load x the request rate plain sustains, submitted to a Server
+    with `workers` device workers; latency = submit -> results on the host.
+    At load 1 the QPS is one the plain path could just keep up with (the
+    latency figure); at load 8 arrivals outpace the server, so the span
+    measures its throughput. An open arrival stream cannot finish faster than
+    it arrives, so throughput is only read at the saturating load."""
+    import openph_amd
+    from openph_amd import complexes as C
+
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    pool = [C.served_request(rng) for _ in range(n_pool)]
+    coo = [b.coo1() for b in pool]
+    gen_s = time.perf_counter() - t0
+    ref = [openph_amd.reduce(b, "pms")[:2] for b in pool]  # warm-up and reference outputs
+    seq = [i % n_pool for i in range(n_requests)]
+    cols = sum(pool[i].m for i in seq)
+    t0 = time.perf_counter()
+    for i in seq:
+        openph_amd.reduce(pool[i], "pms")
+    t_plain = time.perf_counter() - t0
+    runs = {}
+    for load in loads:
+        runs[f"load_{load:g}x"] = _served_run(pool, coo, ref, seq, cols, t_plain, load, workers,
+                                              np.random.default_rng(seed + 1))
+    sat = runs[f"load_{max(loads):g}x"]
+    lat = runs[f"load_{min(loads):g}x"]
+    return {
+        "workload": (f"{n_requests} requests cycling over {n_pool} (seed {seed}): 70% VR "
+                     "N~U{30..200}, 30% cubical side~U{16..96}; Poisson arrivals"),
+        "workers": workers, "columns": cols,
+        "plain_sequential_cols_per_s": cols / t_plain, "plain_sequential_s": t_plain,
+        "served_cols_per_s": sat["served_cols_per_s"],
+        "speedup_served_vs_plain": sat["served_cols_per_s"] / (cols / t_plain),
+        "latency_ms_p50": lat["latency_ms_p50"], "latency_ms_p99": lat["latency_ms_p99"],
+        "qps_for_latency": lat["qps"], "runs": runs,
+        "served_equals_alone": all(r["served_equals_alone"] for r in runs.values()),
+        "host_generation_s": gen_s,
+    }
+
+
+def _served_run(pool, coo, ref, seq, cols, t_plain, load, workers, rng):
+    import threading
+
+    from openph_amd.server import Server
+
+    n_requests = len(seq)
+    qps = load * n_requests / t_plain
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, n_requests))
+    tickets = [None] * n_requests
+    with Server(workers=workers) as srv:
+        def submitter():
+            start = time.perf_counter()
+            for k, i in enumerate(seq):
+                dt = arrivals[k] - (time.perf_counter() - start)
+                if dt > 0:
+                    time.sleep(dt)
+                r, c = coo[i]
+                tickets[k] = srv.submit_coo(r, c, pool[i].m)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat, first, last, same = [], None, None, True
+        for k, i in enumerate(seq):
+            while tickets[k] is None:
+                time.sleep(0.0005)
+            low, ess, info = srv.wait(tickets[k])
+            same &= bool(np.array_equal(low, ref[i][0]) and np.array_equal(ess, ref[i][1]))
+            lat.append(info.done_s - info.submit_s)
+            first = info.submit_s if first is None else min(first, info.submit_s)
+            last = info.done_s if last is None else max(last, info.done_s)
+        th.join()
+        st = srv.stats()
+    span = last - first
+    lat_ms = np.array(lat) * 1e3
+    return {
+        "qps": qps, "offered_load_vs_plain": load,
+        "served_cols_per_s": cols / span, "served_span_s": span,
+        "latency_ms_p50": float(np.percentile(lat_ms, 50)),
+        "latency_ms_p99": float(np.percentile(lat_ms, 99)),
+        "latency_ms_max": float(lat_ms.max()),
+        "batches": st["batches"], "served_equals_alone": same,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +237,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy, the reference kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -245,6 +339,9 @@ def main():
     if not args.no_batched:
         print("bench: batched leg", file=sys.stderr, flush=True)
         out["batched"] = batched_leg()
+    if not args.no_served:
+        print("bench: served leg", file=sys.stderr, flush=True)
+        out["served"] = served_leg()
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         cb, cpu_low = cpu_baseline(b)

@@ -99,6 +99,7 @@ typedef struct openph_stats {
     int lanes;               /* Phase II lanes per column that were used     */
     int spills;              /* columns whose additions left LDS             */
     int group_lds;           /* LDS entries per lane-group buffer used       */
+    int handoffs;            /* columns continued by the block kernel        */
 } openph_stats;
 
 /*
@@ -139,6 +140,50 @@ int openph_pms_batch(int count, const int *ms, const int *nnzs, const int *rows1
 int openph_pms_batch_device(int count, const int *ms, const int *nnzs, const int *rows1,
                             const int *cols1, int *low1, int *ess, void *stream,
                             const openph_options *opt, openph_stats *stats);
+
+/*
+ * Served reduction (no counterpart in the reference, which is a blocking
+ * MATLAB call). A server owns `workers` device workspaces, each with its own
+ * HIP stream and thread; queued requests are batched onto a free worker while
+ * their pool footprint fits worker_bytes, and batches of different workers
+ * run concurrently on the device.
+ */
+typedef struct openph_server openph_server;
+
+typedef struct openph_server_options {
+    int device;              /* HIP device ordinal                           */
+    int workers;             /* concurrent batches (0 = 4, at most 16)       */
+    int64_t worker_bytes;    /* device bytes a batch may claim (0 = 8 GiB)   */
+    int max_batch;           /* requests per batch (0 = 1024)                */
+    int max_queue;           /* queued requests before OPENPH_EBUSY
+                                (0 = 65536)                                  */
+    openph_options reduce;   /* options of each reduction                    */
+} openph_server_options;
+
+typedef struct openph_request_info {
+    double submit_s;         /* seconds on the server's steady clock         */
+    double start_s;          /* its batch started                            */
+    double done_s;           /* its batch finished (results on the host)     */
+    int batch_size;          /* requests in its batch                        */
+    int worker;              /* worker that ran it                           */
+    int num_iters;           /* num_iters of its batch                       */
+    int status;              /* 0 or a negative OPENPH_E* code               */
+    int m;                   /* columns of the request                       */
+} openph_request_info;
+
+int openph_server_create(const openph_server_options *opt, openph_server **out);
+/* Queue one matrix (host 1-based COO, copied); returns a ticket. Never waits
+ * for the device; OPENPH_EBUSY when the queue is full. */
+int openph_server_submit(openph_server *srv, const int *rows1, const int *cols1, int m, int nnz,
+                         int64_t *ticket);
+/* Block until the ticket's reduction is done; low1/ess (host, m entries) get
+ * its outputs. A ticket can be waited for once. */
+int openph_server_wait(openph_server *srv, int64_t ticket, int *low1, int *ess,
+                       openph_request_info *info);
+int openph_server_stats(openph_server *srv, int64_t *queued, int64_t *batches,
+                        int64_t *completed);
+/* Finishes queued requests, then stops the workers and frees the server. */
+int openph_server_destroy(openph_server *srv);
 
 /* Human-readable message for a status code. */
 const char *openph_strerror(int status);

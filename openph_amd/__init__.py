@@ -73,6 +73,7 @@ class Stats(ctypes.Structure):
         ("lanes", ctypes.c_int),
         ("spills", ctypes.c_int),
         ("group_lds", ctypes.c_int),
+        ("handoffs", ctypes.c_int),
     ]
 
     def as_dict(self):

@@ -28,6 +28,8 @@ struct Ctrl {
     int marks;                     // columns marked by Phase I (cumulative)
     int nwork;                     // columns queued for Phase II
     int spills;                    // Phase II columns that left LDS (cumulative)
+    int nlong;                     // columns handed to the block kernel
+    int handoffs;                  // the same, cumulative
 };
 
 // Validation error bits (Ctrl::err).
@@ -50,6 +52,7 @@ struct Dev {
     int *left = nullptr, *beta = nullptr, *first = nullptr, *clear = nullptr;
     int *newpiv = nullptr, *slot = nullptr, *len = nullptr, *cap = nullptr;
     int *work = nullptr;                     // Phase II worklist
+    int *longw = nullptr;                    // columns handed to the block kernel
     int4 *pivdesc = nullptr;                 // per row: {pivot column, off, len, 0}
     unsigned *off = nullptr, *off2 = nullptr;
     unsigned long long *scan_tmp = nullptr;  // tiled-scan partials
@@ -70,9 +73,9 @@ void launch_phase1_detect(const Dev &d, hipStream_t s);
 void launch_phase1_mark(const Dev &d, hipStream_t s);
 void launch_apply1(const Dev &d, hipStream_t s);
 // lanes: 1 = one lane per column over all columns; 4..64 = lane groups over
-// the worklist with wlds LDS entries per buffer. grid = blocks for the group
-// kernels.
-void launch_phase2(const Dev &d, int lanes, int wlds, int grid, hipStream_t s);
+// the worklist with wlds LDS entries per buffer, followed by the block kernel
+// for the columns they hand off. grid / long_grid = blocks of the two.
+void launch_phase2(const Dev &d, int lanes, int wlds, int grid, int long_grid, hipStream_t s);
 // Batches: local 1-based COO of `count` matrices -> global block-diagonal
 // COO (err gets bit 1 for an index outside its matrix), and global 1-based
 // low -> local 1-based low.

@@ -201,7 +201,9 @@ __global__ void k_ctrl_init(Ctrl *c, unsigned long long cap) {
         c->maxlen = 0;
         c->marks = 0;
         c->nwork = 0;
+        c->nlong = 0;
         c->spills = 0;
+        c->handoffs = 0;
     }
 }
 
@@ -410,6 +412,7 @@ __global__ void k_reset_flags(Ctrl *c) {
         c->any_dup = 0;
         c->overflow = 0;
         c->nwork = 0;
+        c->nlong = 0;
     }
 }
 
@@ -766,22 +769,201 @@ __device__ __forceinline__ int mp_walk(const int *a, int na, const int *b, int n
     return cnt;
 }
 
+// A team merges one column: G lanes of one wave (WaveTeam) or a whole block
+// (BlockTeam). It provides its lane index and size, a barrier, an exclusive
+// scan with total, a broadcast from its leader and pool allocation.
 template <int G>
-__device__ int group_xor(const int *a, int na, const int *b, int nb, int *c, int gl) {
+struct WaveTeam {
+    static constexpr int size = G;
+    int gl, gw;
+    __device__ int lane() const { return gl; }
+    __device__ void sync() const { __syncwarp(); }
+    // stores of the team to global memory visible to the team's later loads
+    __device__ void fence() const { __threadfence_block(); }
+    __device__ int scan(int v, int &total) const {
+        int incl = v;
+#pragma unroll
+        for (int o = 1; o < G; o <<= 1) {
+            const int t = __shfl_up(incl, o, G);
+            if (gl >= o) incl += t;
+        }
+        total = __shfl(incl, G - 1, G);
+        return incl - v;
+    }
+    __device__ unsigned long long bcast(unsigned long long x) const {
+        return __shfl(x, gw * G, 64);
+    }
+};
+
+struct BlockTeam {
+    static constexpr int size = kBlock;
+    int *s_scan;                 // kBlock / 64 ints of LDS
+    unsigned long long *s_word;  // one word of LDS
+    __device__ int lane() const { return threadIdx.x; }
+    __device__ void sync() const { __syncthreads(); }
+    __device__ void fence() const {
+        __threadfence_block();
+        __syncthreads();
+    }
+    __device__ int scan(int v, int &total) const {
+        return block_exclusive_scan(v, SumOp<int>(), 0, s_scan, &total);
+    }
+    __device__ unsigned long long bcast(unsigned long long x) const {
+        if (threadIdx.x == 0) *s_word = x;
+        __syncthreads();
+        const unsigned long long v = *s_word;
+        __syncthreads();
+        return v;
+    }
+};
+
+template <class Team>
+__device__ int team_xor(const Team &t, const int *a, int na, const int *b, int nb, int *c) {
     const int n = na + nb;
-    const int K = (n + G - 1) / G;
-    const int d0 = gl * K < n ? gl * K : n;
+    const int K = (n + Team::size - 1) / Team::size;
+    const int l = t.lane();
+    const int d0 = l * K < n ? l * K : n;
     const int d1 = d0 + K < n ? d0 + K : n;
     const int cnt = mp_walk<false>(a, na, b, nb, d0, d1, nullptr);
-    int incl = cnt;
-#pragma unroll
-    for (int o = 1; o < G; o <<= 1) {
-        const int t = __shfl_up(incl, o, G);
-        if (gl >= o) incl += t;
-    }
-    const int total = __shfl(incl, G - 1, G);
-    mp_walk<true>(a, na, b, nb, d0, d1, c + (incl - cnt));
+    int total;
+    const int ex = t.scan(cnt, total);
+    mp_walk<true>(a, na, b, nb, d0, d1, c + ex);
     return total;
+}
+
+// Allocate `n` pool entries by the team leader; ~0ull when the pool is full.
+template <class Team>
+__device__ __forceinline__ unsigned long long team_alloc(const Team &t, const Dev &d, unsigned n) {
+    unsigned long long base = 0;
+    if (t.lane() == 0) {
+        base = atomicAdd(&d.ctrl->pool_top, (unsigned long long)n);
+        if (base + n > d.ctrl->pool_cap) base = ~0ull;
+    }
+    return t.bcast(base);
+}
+
+struct Phase2Counters {
+    unsigned long long nadd = 0, nel = 0;
+    int spills = 0;
+    bool ovf = false;
+};
+
+// The Phase II chain of column j (pms_functions.cu:87-112) by one team. The
+// column's contents stay in LDS buffer sB (W entries) while they fit; a
+// pivot column is staged into sA and the sum goes to sC. With HANDOFF, a
+// column whose next sum may not fit W is committed as it stands and queued
+// on the long list for the block kernel, which continues the same chain;
+// without it, the team merges straight in the pool.
+template <class Team, int W, bool HANDOFF>
+__device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phase2Counters &ctr) {
+    const int gl = t.lane();
+    constexpr int G = Team::size;
+    int lj = d.low[j];
+    int4 pd = d.pivdesc[lj];
+    unsigned coff = d.off[j];
+    int clen = d.len[j], ccap = d.cap[j];
+    int *sA = buf, *sB = buf + W, *sC = buf + 2 * W;
+    bool glob = clen > W;  // contents in the pool instead of LDS
+    bool handoff = HANDOFF && glob;
+    if (!glob)
+        for (int i = gl; i < clen; i += G) sB[i] = d.pool[coff + i];
+    t.sync();
+    bool dirty = false, becomes_pivot = false, lost = false;
+    while (!handoff && pd.x > -1 && pd.x < j) {
+        const int na = pd.z;
+        const unsigned aoff = (unsigned)pd.y;
+        int nc, nl;
+        if (!glob && na + clen <= W) {
+            if (gl == 0) d.ess[lj] = 0;
+            for (int i = gl; i < na; i += G) sA[i] = d.pool[aoff + i];
+            t.sync();
+            nc = team_xor(t, sA, na, sB, clen, sC);
+            t.sync();
+            int *tmp = sB;
+            sB = sC;
+            sC = tmp;
+            nl = nc ? sB[nc - 1] : -1;
+        } else if (HANDOFF) {
+            handoff = true;
+            break;
+        } else {
+            if (gl == 0) d.ess[lj] = 0;
+            // the result may not fit in LDS: merge straight into a fresh pool
+            // segment, reading the current contents from LDS the first time
+            // and from the pool afterwards
+            const unsigned long long base = team_alloc(t, d, (unsigned)(na + clen));
+            if (base == ~0ull) {
+                ctr.ovf = true;
+                lost = !glob && dirty;  // progress held only in LDS
+                break;
+            }
+            const int *cur = glob ? d.pool + coff : sB;
+            nc = team_xor(t, d.pool + aoff, na, cur, clen, d.pool + base);
+            t.fence();
+            if (!glob) ctr.spills++;
+            glob = true;
+            coff = (unsigned)base;
+            ccap = na + clen;
+            nl = nc ? d.pool[base + nc - 1] : -1;
+        }
+        ctr.nadd++;
+        ctr.nel += (unsigned long long)(na + clen + nc);
+        clen = nc;
+        dirty = true;
+        if (nl >= lj) {
+            if (gl == 0) atomicOr(&d.ctrl->err, kErrState);
+            break;
+        }
+        lj = nl;
+        if (lj < 0) break;
+        if (gl == 0) d.clear[lj] = 1;
+        if (d.left[lj] == j) {
+            becomes_pivot = true;
+            break;
+        }
+        pd = d.pivdesc[lj];
+    }
+    bool committed = false;
+    if (!lost && dirty) {
+        bool ok = true;
+        if (!glob) {
+            // write back from LDS: in place when the segment is big enough
+            if (clen > ccap) {
+                unsigned want = (unsigned)clen;
+                if (want < (unsigned)d.slot0) want = (unsigned)d.slot0;
+                want = (want + 7u) & ~7u;
+                const unsigned long long base = team_alloc(t, d, want);
+                if (base == ~0ull) {
+                    ctr.ovf = true;
+                    ok = false;  // nothing of this column's progress is kept
+                } else {
+                    coff = (unsigned)base;
+                    ccap = (int)want;
+                }
+            }
+            if (ok)
+                for (int i = gl; i < clen; i += G) d.pool[coff + i] = sB[i];
+        }
+        if (ok && gl == 0) {
+            d.off[j] = coff;
+            d.len[j] = clen;
+            d.cap[j] = ccap;
+            d.low[j] = lj;
+            if (becomes_pivot) {
+                d.newpiv[lj] = j;
+                d.classes[j] = -1;
+            }
+        }
+        committed = ok;
+    }
+    // a handed-off column continues in the block kernel from the state just
+    // committed (or from its unchanged state)
+    if (HANDOFF && handoff && (committed || !dirty) && gl == 0) {
+        const int k = atomicAdd(&d.ctrl->nlong, 1);
+        d.longw[k] = j;
+        atomicAdd(&d.ctrl->handoffs, 1);
+    }
+    t.sync();  // the buffers are reused by the team's next column
 }
 
 // Lane-group shapes: G lanes per column, W LDS entries per buffer (three
@@ -793,142 +975,55 @@ struct GroupShape {
     static_assert(LDS_BYTES <= 64 * 1024, "static LDS limit");
 };
 
-// Allocate `n` pool entries by the group leader; broadcast to the group.
-// Returns ~0ull when the pool is full.
-template <int G>
-__device__ __forceinline__ unsigned long long group_alloc(const Dev &d, unsigned n, int gl, int gw) {
-    unsigned long long base = 0;
-    if (gl == 0) {
-        base = atomicAdd(&d.ctrl->pool_top, (unsigned long long)n);
-        if (base + n > d.ctrl->pool_cap) base = ~0ull;
-    }
-    return __shfl(base, gw * G, 64);
-}
-
 template <int G, int W>
 __global__ void __launch_bounds__(kBlock) k_phase2_group(Dev d) {
     constexpr int NG = GroupShape<G, W>::NG;
     __shared__ int lds[NG * 3 * W];
+    const WaveTeam<G> t{(int)(threadIdx.x % G), (int)((threadIdx.x & 63) / G)};
     const int gid = threadIdx.x / G;
-    const int gl = threadIdx.x % G;
-    const int gw = (threadIdx.x & 63) / G;
     int *const buf = lds + gid * 3 * W;
     const int nwork = d.ctrl->nwork;
-    const int slot0 = d.slot0;
-    unsigned long long nadd = 0, nel = 0;
-    int spills = 0;
-    bool ovf = false;
-
-    for (int w = blockIdx.x * NG + gid; w < nwork; w += gridDim.x * NG) {
-        const int j = d.work[w];
-        int lj = d.low[j];
-        int4 pd = d.pivdesc[lj];
-        unsigned coff = d.off[j];
-        int clen = d.len[j], ccap = d.cap[j];
-        int *sA = buf, *sB = buf + W, *sC = buf + 2 * W;
-        bool glob = clen > W;  // contents in the pool instead of LDS
-        if (!glob)
-            for (int i = gl; i < clen; i += G) sB[i] = d.pool[coff + i];
-        __syncwarp();
-        bool dirty = false, becomes_pivot = false, lost = false;
-        while (pd.x > -1 && pd.x < j) {
-            const int na = pd.z;
-            const unsigned aoff = (unsigned)pd.y;
-            if (gl == 0) d.ess[lj] = 0;
-            int nc, nl;
-            if (!glob && na + clen <= W) {
-                for (int i = gl; i < na; i += G) sA[i] = d.pool[aoff + i];
-                __syncwarp();
-                nc = group_xor<G>(sA, na, sB, clen, sC, gl);
-                __syncwarp();
-                int *t = sB;
-                sB = sC;
-                sC = t;
-                nl = nc ? sB[nc - 1] : -1;
-            } else {
-                // The result may not fit in LDS: the group merges straight
-                // into a fresh pool segment, reading the current contents from
-                // LDS the first time and from the pool afterwards.
-                unsigned long long base = group_alloc<G>(d, (unsigned)(na + clen), gl, gw);
-                if (base == ~0ull) {
-                    ovf = true;
-                    lost = !glob && dirty;  // progress held only in LDS
-                    break;
-                }
-                const int *cur = glob ? d.pool + coff : sB;
-                nc = group_xor<G>(d.pool + aoff, na, cur, clen, d.pool + base, gl);
-                // the group's stores must be visible to its own later loads
-                __threadfence_block();
-                if (!glob) spills++;
-                glob = true;
-                coff = (unsigned)base;
-                ccap = na + clen;
-                nl = nc ? d.pool[base + nc - 1] : -1;
-            }
-            nadd++;
-            nel += (unsigned long long)(na + clen + nc);
-            clen = nc;
-            dirty = true;
-            if (nl >= lj) {
-                if (gl == 0) atomicOr(&d.ctrl->err, kErrState);
-                break;
-            }
-            lj = nl;
-            if (lj < 0) break;
-            if (gl == 0) d.clear[lj] = 1;
-            if (d.left[lj] == j) {
-                becomes_pivot = true;
-                break;
-            }
-            pd = d.pivdesc[lj];
-        }
-        if (lost || !dirty) continue;
-        if (!glob) {
-            // write back from LDS: in place when the segment is big enough
-            if (clen > ccap) {
-                unsigned want = (unsigned)clen;
-                if (want < (unsigned)slot0) want = (unsigned)slot0;
-                want = (want + 7u) & ~7u;
-                unsigned long long base = group_alloc<G>(d, want, gl, gw);
-                if (base == ~0ull) {
-                    ovf = true;
-                    continue;  // nothing of this column's progress is kept
-                }
-                coff = (unsigned)base;
-                ccap = (int)want;
-            }
-            for (int i = gl; i < clen; i += G) d.pool[coff + i] = sB[i];
-        }
-        if (gl == 0) {
-            d.off[j] = coff;
-            d.len[j] = clen;
-            d.cap[j] = ccap;
-            d.low[j] = lj;
-            if (becomes_pivot) {
-                d.newpiv[lj] = j;
-                d.classes[j] = -1;
-            }
-        }
-    }
-    if (ovf && gl == 0) d.ctrl->overflow = 1;
+    Phase2Counters ctr;
+    for (int w = blockIdx.x * NG + gid; w < nwork; w += gridDim.x * NG)
+        phase2_column<WaveTeam<G>, W, true>(t, d, d.work[w], buf, ctr);
+    if (ctr.ovf && t.gl == 0) d.ctrl->overflow = 1;
     // per-group counters are identical on the group's lanes: count leaders only
-    if (gl != 0) {
-        nadd = 0;
-        nel = 0;
-        spills = 0;
-    }
+    if (t.gl != 0) ctr = Phase2Counters();
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        nadd += __shfl_down(nadd, o, 64);
-        nel += __shfl_down(nel, o, 64);
-        spills += __shfl_down(spills, o, 64);
+        ctr.nadd += __shfl_down(ctr.nadd, o, 64);
+        ctr.nel += __shfl_down(ctr.nel, o, 64);
+        ctr.spills += __shfl_down(ctr.spills, o, 64);
     }
     if (lane_id() == 0) {
-        if (nadd) {
-            atomicAdd(&d.ctrl->additions, nadd);
-            atomicAdd(&d.ctrl->merge_elems, nel);
+        if (ctr.nadd) {
+            atomicAdd(&d.ctrl->additions, ctr.nadd);
+            atomicAdd(&d.ctrl->merge_elems, ctr.nel);
         }
-        if (spills) atomicAdd(&d.ctrl->spills, spills);
+        if (ctr.spills) atomicAdd(&d.ctrl->spills, ctr.spills);
+    }
+}
+
+// Columns handed off by the lane groups: one block (256 lanes) per column,
+// 4096-entry LDS buffers, merging straight in the pool beyond that.
+constexpr int kLongW = 4096;
+
+__global__ void __launch_bounds__(kBlock) k_phase2_long(Dev d) {
+    __shared__ int lds[3 * kLongW];
+    __shared__ int s_scan[kBlock / 64];
+    __shared__ unsigned long long s_word;
+    const BlockTeam t{s_scan, &s_word};
+    const int nlong = d.ctrl->nlong;
+    Phase2Counters ctr;
+    for (int w = blockIdx.x; w < nlong; w += gridDim.x)
+        phase2_column<BlockTeam, kLongW, false>(t, d, d.longw[w], lds, ctr);
+    if (threadIdx.x == 0) {
+        if (ctr.ovf) d.ctrl->overflow = 1;
+        if (ctr.nadd) {
+            atomicAdd(&d.ctrl->additions, ctr.nadd);
+            atomicAdd(&d.ctrl->merge_elems, ctr.nel);
+        }
+        if (ctr.spills) atomicAdd(&d.ctrl->spills, ctr.spills);
     }
 }
 
@@ -979,7 +1074,7 @@ int phase2_blocks_per_cu(int lanes, int wlds) {
     return n;
 }
 
-void launch_phase2(const Dev &d, int lanes, int wlds, int grid, hipStream_t s) {
+void launch_phase2(const Dev &d, int lanes, int wlds, int grid, int long_grid, hipStream_t s) {
     Phase2Kernel k = phase2_kernel(lanes, wlds);
     if (!k) {
         int gm = (d.m + kBlock - 1) / kBlock;
@@ -987,6 +1082,7 @@ void launch_phase2(const Dev &d, int lanes, int wlds, int grid, hipStream_t s) {
         return;
     }
     k<<<grid, kBlock, 0, s>>>(d);
+    k_phase2_long<<<long_grid, kBlock, 0, s>>>(d);
 }
 
 // After Phase II: publish new pivots, clear_positives, ess_hat

@@ -22,69 +22,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/openph.h"
-#include "engine.h"
+#include "host.h"
 
 namespace oph {
-
-#define OPH_CHECK(expr)                                  \
-    do {                                                 \
-        hipError_t e_ = (expr);                          \
-        if (e_ != hipSuccess) {                          \
-            last_hip_error() = e_;                       \
-            return OPENPH_EDEVICE;                       \
-        }                                                \
-    } while (0)
-
-static hipError_t &last_hip_error() {
-    static thread_local hipError_t e = hipSuccess;
-    return e;
-}
-
-template <typename T>
-static hipError_t grow(T **p, size_t &cap, size_t need) {
-    if (need <= cap && *p) return hipSuccess;
-    if (*p) {
-        hipError_t e = hipFree(*p);
-        if (e != hipSuccess) return e;
-        *p = nullptr;
-    }
-    size_t n = std::max<size_t>(need, 1);
-    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
-    if (e != hipSuccess) {
-        cap = 0;
-        *p = nullptr;
-        return e;
-    }
-    cap = n;
-    return hipSuccess;
-}
-
-// Device workspace of one GPU, reused by every call on that GPU.
-struct Context {
-    int device = 0;
-    std::mutex mu;
-    Dev d;
-    size_t cap_m = 0, cap_tiles = 0, cap_pool = 0, cap_pool2 = 0, cap_slotmap = 0;
-    size_t cap_scan = 0;
-    // per-column arrays share cap_m; remember each allocation separately
-    size_t c_low = 0, c_arglow = 0, c_classes = 0, c_ess = 0, c_left = 0, c_beta = 0,
-           c_first = 0, c_clear = 0, c_newpiv = 0, c_slot = 0, c_len = 0, c_off = 0,
-           c_off2 = 0, c_tmax = 0, c_tpre = 0, c_ctrl = 0, c_cap = 0, c_work = 0, c_pivdesc = 0;
-    int num_cu = 0;
-    // staging for host-pointer entry points
-    int *h_rows = nullptr, *h_cols = nullptr, *h_low = nullptr, *h_ess = nullptr;
-    size_t c_hrows = 0, c_hcols = 0, c_hlow = 0, c_hess = 0;
-    // batches: per-matrix offsets and the block-diagonal COO
-    int *b_moff = nullptr, *b_noff = nullptr, *b_rows = nullptr, *b_cols = nullptr;
-    int *b_err = nullptr;
-    size_t c_bmoff = 0, c_bnoff = 0, c_brows = 0, c_bcols = 0, c_berr = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::vector<hipEvent_t> iter_events;
-    std::vector<hipEvent_t> p2_events;  // start/end pairs of Phase II launches
-    bool init = false;
-};
 
 static Context *context_for(int device) {
     static std::mutex gmu;
@@ -118,6 +58,7 @@ static int ensure(Context &c, int m, int nnz, size_t pool_elems) {
     OPH_CHECK(grow(&d.cap, c.c_cap, mm));
     OPH_CHECK(grow(&d.work, c.c_work, mm));
     OPH_CHECK(grow(&d.pivdesc, c.c_pivdesc, mm));
+    OPH_CHECK(grow(&d.longw, c.c_longw, mm));
     OPH_CHECK(grow(&d.slotmap, c.cap_slotmap, mm + 1));
     size_t nt = (size_t)(m + kTile - 1) / kTile + 1;
     OPH_CHECK(grow(&d.tile_max, c.c_tmax, nt * kMaxSlots));
@@ -130,18 +71,19 @@ static int ensure(Context &c, int m, int nnz, size_t pool_elems) {
     return OPENPH_OK;
 }
 
-struct RunCfg {
-    int slot0 = 32;
-    int max_iters = 1000000;
-    size_t pool_elems = 0;
-    size_t pool_max = 0xFFFFFFFFull;
-    int lanes = 0;
-    int wlds = 0;
-    bool track = false;
-    bool timing = true;
-};
+int reserve(Context &c, size_t m, size_t nnz, size_t pool_elems) {
+    if (m > 0x7ffffffeull || nnz > 0x7ffffffeull) return OPENPH_EINVAL;
+    int rc = ensure(c, (int)m, (int)nnz, pool_elems);
+    if (rc) return rc;
+    OPH_CHECK(grow(&c.h_rows, c.c_hrows, nnz + 1));
+    OPH_CHECK(grow(&c.h_cols, c.c_hcols, nnz + 1));
+    OPH_CHECK(grow(&c.b_rows, c.c_brows, nnz + 1));
+    OPH_CHECK(grow(&c.b_cols, c.c_bcols, nnz + 1));
+    OPH_CHECK(grow(&c.b_err, c.c_berr, 1));
+    return OPENPH_OK;
+}
 
-static RunCfg cfg_from(const openph_options *opt, int nnz) {
+RunCfg cfg_from(const openph_options *opt, int nnz) {
     RunCfg r;
     (void)nnz;
     r.pool_elems = 0;  // automatic: the initial layout plus headroom
@@ -295,10 +237,10 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
                 c.p2_events.push_back(e);
             }
             OPH_CHECK(hipEventRecord(c.p2_events[k], s));
-            launch_phase2(d, lanes, wlds, grid, s);
+            launch_phase2(d, lanes, wlds, grid, c.num_cu, s);
             OPH_CHECK(hipEventRecord(c.p2_events[k + 1], s));
         } else {
-            launch_phase2(d, lanes, wlds, grid, s);
+            launch_phase2(d, lanes, wlds, grid, c.num_cu, s);
         }
         launch_apply2(d, s);
         if (time_track && (size_t)iter <= (size_t)m) {
@@ -350,6 +292,7 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
         stats->lanes = lanes;
         stats->group_lds = wlds;
         stats->spills = h.spills;
+        stats->handoffs = h.handoffs;
         stats->phase2_launches = iter - 1;
         float p2 = 0.f;
         for (int i = 0; i < iter - 1; i++) {
@@ -378,7 +321,7 @@ static bool known_algo(const char *a) {
     return false;
 }
 
-static int set_device(int device) {
+int set_device(int device) {
     int n = 0;
     OPH_CHECK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return OPENPH_EINVAL;
@@ -425,7 +368,7 @@ static int run_host(const char *algstr, const int *rows1, const int *cols1, int 
 // Batch of `count` matrices (device COO, local 1-based indices) reduced as
 // one block-diagonal matrix. low1/ess (device) receive the concatenated
 // per-matrix outputs with local 1-based lows.
-static int run_batch(Context &c, int count, const int *ms, const int *nnzs, const int *rows1,
+int run_batch(Context &c, int count, const int *ms, const int *nnzs, const int *rows1,
                      const int *cols1, int *low1, int *ess, hipStream_t s, const RunCfg &cfg,
                      openph_stats *stats) {
     if (count < 0 || (count > 0 && (!ms || !nnzs))) return OPENPH_EINVAL;

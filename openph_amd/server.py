@@ -1,0 +1,139 @@
+"""Served pms: Python handle over the openph_server C ABI (include/openph.h).
+
+    with Server(workers=4) as srv:
+        t = srv.submit(boundary)          # returns at once
+        low0, ess, info = srv.wait(t)     # blocks until that request is done
+
+ctypes releases the GIL during foreign calls, so submit() and wait() can be
+called from different Python threads (an arrival loop and a collector).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import Options, OpenPHError, lib
+
+__all__ = ["Server", "ServerOptions", "RequestInfo"]
+
+
+class ServerOptions(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int),
+        ("workers", ctypes.c_int),
+        ("worker_bytes", ctypes.c_int64),
+        ("max_batch", ctypes.c_int),
+        ("max_queue", ctypes.c_int),
+        ("reduce", Options),
+    ]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("submit_s", ctypes.c_double),
+        ("start_s", ctypes.c_double),
+        ("done_s", ctypes.c_double),
+        ("batch_size", ctypes.c_int),
+        ("worker", ctypes.c_int),
+        ("num_iters", ctypes.c_int),
+        ("status", ctypes.c_int),
+        ("m", ctypes.c_int),
+    ]
+
+    @property
+    def latency_s(self) -> float:
+        return self.done_s - self.submit_s
+
+
+def _bind():
+    L = lib()
+    if getattr(L, "_server_bound", False):
+        return L
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.openph_server_create.restype = ctypes.c_int
+    L.openph_server_create.argtypes = [ctypes.POINTER(ServerOptions), ctypes.POINTER(ctypes.c_void_p)]
+    L.openph_server_submit.restype = ctypes.c_int
+    L.openph_server_submit.argtypes = [ctypes.c_void_p, ip, ip, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_int64)]
+    L.openph_server_wait.restype = ctypes.c_int
+    L.openph_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_int64, ip, ip,
+                                     ctypes.POINTER(RequestInfo)]
+    L.openph_server_stats.restype = ctypes.c_int
+    L.openph_server_stats.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int64)] * 3
+    L.openph_server_destroy.restype = ctypes.c_int
+    L.openph_server_destroy.argtypes = [ctypes.c_void_p]
+    L._server_bound = True
+    return L
+
+
+class Server:
+    def __init__(self, workers: int = 4, worker_bytes: int = 8 << 30, max_batch: int = 1024,
+                 max_queue: int = 1 << 16, device: int = 0, reduce: Options | None = None):
+        L = _bind()
+        o = ServerOptions()
+        o.device = device
+        o.workers = workers
+        o.worker_bytes = worker_bytes
+        o.max_batch = max_batch
+        o.max_queue = max_queue
+        if reduce is not None:
+            o.reduce = reduce
+        h = ctypes.c_void_p()
+        rc = L.openph_server_create(ctypes.byref(o), ctypes.byref(h))
+        if rc != 0:
+            raise OpenPHError(rc, "openph_server_create")
+        self._h = h
+        self._m = {}
+
+    def submit_coo(self, rows1, cols1, m: int) -> int:
+        rows1 = np.ascontiguousarray(rows1, dtype=np.int32)
+        cols1 = np.ascontiguousarray(cols1, dtype=np.int32)
+        t = ctypes.c_int64()
+        ip = ctypes.POINTER(ctypes.c_int)
+        rc = _bind().openph_server_submit(self._h, rows1.ctypes.data_as(ip),
+                                          cols1.ctypes.data_as(ip), int(m), int(rows1.shape[0]),
+                                          ctypes.byref(t))
+        if rc != 0:
+            raise OpenPHError(rc, "openph_server_submit")
+        self._m[t.value] = int(m)
+        return t.value
+
+    def submit(self, boundary) -> int:
+        r, c = boundary.coo1()
+        return self.submit_coo(r, c, boundary.m)
+
+    def wait(self, ticket: int):
+        """Returns (low0, ess, RequestInfo); low0 is 0-based (-1 = zero)."""
+        m = self._m.pop(ticket)
+        low = np.empty(m, np.int32)
+        ess = np.empty(m, np.int32)
+        info = RequestInfo()
+        ip = ctypes.POINTER(ctypes.c_int)
+        rc = _bind().openph_server_wait(self._h, ticket, low.ctypes.data_as(ip),
+                                        ess.ctypes.data_as(ip), ctypes.byref(info))
+        if rc != 0:
+            raise OpenPHError(rc, "openph_server_wait")
+        return low - 1, ess, info
+
+    def stats(self):
+        q, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _bind().openph_server_stats(self._h, ctypes.byref(q), ctypes.byref(b), ctypes.byref(c))
+        return {"queued": q.value, "batches": b.value, "completed": c.value}
+
+    def close(self):
+        if self._h:
+            _bind().openph_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,104 @@
+"""Served pms on the GPU: results in a concurrent mix equal each request
+reduced alone (and the golden fixtures / oracle), from several submitting
+threads, with requests of very different sizes."""
+import json
+import os
+import sys
+import threading
+
+import numpy as np
+import pytest
+
+import oracle
+import openph_amd
+from openph_amd import complexes as C
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+INDEX = json.load(open(os.path.join(GOLD, "index.json")))
+sys.path.insert(0, os.path.dirname(__file__))
+from test_oracle import _gen  # noqa: E402
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    openph_amd.lib()
+
+
+def test_server_mix_equals_alone():
+    from openph_amd.server import Server
+    print("server mix", flush=True)
+    names = sorted(INDEX)
+    mats = [_gen(n) for n in names] + C.batched_mix(12, seed=11) + [C.cubical64(seed=3, side=20)]
+    alone = [openph_amd.reduce(b, "pms")[:2] for b in mats]
+    with Server(workers=3, worker_bytes=256 << 20) as srv:
+        tickets = [None] * len(mats)
+
+        def submit(idx):
+            for i in idx:
+                tickets[i] = srv.submit(mats[i])
+
+        th = [threading.Thread(target=submit, args=(list(range(k, len(mats), 3)),))
+              for k in range(3)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        infos = []
+        for i, b in enumerate(mats):
+            low, ess, info = srv.wait(tickets[i])
+            infos.append(info)
+            assert info.status == 0 and info.m == b.m
+            assert np.array_equal(low, alone[i][0]), b.name
+            assert np.array_equal(ess, alone[i][1]), b.name
+            if i < len(names):
+                f = np.load(os.path.join(GOLD, names[i] + ".npz"))
+                assert np.array_equal(low, f["low"])
+        st = srv.stats()
+    assert st["completed"] == len(mats)
+    assert all(i.done_s >= i.start_s >= i.submit_s for i in infos)
+    # the requests were batched: fewer batches than requests
+    assert st["batches"] < len(mats)
+
+
+def test_server_small_budget_runs_oversized_request_alone():
+    from openph_amd.server import Server
+    print("server budget", flush=True)
+    big = C.torus100()
+    small = [_gen("vr12"), _gen("random1")]
+    with Server(workers=1, worker_bytes=1 << 20) as srv:
+        tb = srv.submit(big)
+        ts = [srv.submit(b) for b in small]
+        low, ess, info = srv.wait(tb)
+        f = np.load(os.path.join(GOLD, "torus100.npz"))
+        assert np.array_equal(low, f["low"]) and info.batch_size == 1
+        for b, t in zip(small, ts):
+            low, ess, _ = srv.wait(t)
+            assert np.array_equal(low, oracle.pms(b).low)
+
+
+def test_server_rejects_malformed_request_without_poisoning_others():
+    from openph_amd.server import Server
+    print("server malformed", flush=True)
+    r = np.array([1, 2, 1, 3, 2, 3, 4, 5, 6], np.int32)
+    c = np.array([4, 4, 5, 5, 6, 6, 7, 7, 7], np.int32)
+    bad_r = r.copy()
+    bad_r[0] = 5  # row >= col
+    with Server(workers=1) as srv:
+        tg0 = srv.submit_coo(r, c, 7)
+        with pytest.raises(openph_amd.OpenPHError) as e:
+            srv.submit_coo(bad_r, c, 7)
+        assert e.value.status == -1
+        bad_c = c.copy()
+        bad_c[2], bad_c[6] = bad_c[6], bad_c[2]
+        with pytest.raises(openph_amd.OpenPHError) as e:
+            srv.submit_coo(r, bad_c, 7)
+        assert e.value.status == -2
+        tg = srv.submit_coo(r, c, 7)
+        for t in (tg0, tg):
+            low, ess, info = srv.wait(t)
+            assert (low + 1).tolist() == [0, 0, 0, 2, 3, 0, 6]
