@@ -795,9 +795,10 @@ struct WaveTeam {
     }
 };
 
+template <int BT>
 struct BlockTeam {
-    static constexpr int size = kBlock;
-    int *s_scan;                 // kBlock / 64 ints of LDS
+    static constexpr int size = BT;
+    int *s_scan;                 // BT / 64 ints of LDS
     unsigned long long *s_word;  // one word of LDS
     __device__ int lane() const { return threadIdx.x; }
     __device__ void sync() const { __syncthreads(); }
@@ -828,6 +829,26 @@ __device__ int team_xor(const Team &t, const int *a, int na, const int *b, int n
     int total;
     const int ex = t.scan(cnt, total);
     mp_walk<true>(a, na, b, nb, d0, d1, c + ex);
+    return total;
+}
+
+// The same sum with one walk, for operands in LDS: each lane writes its
+// survivors to c at its first diagonal (a provisional slot, never beyond its
+// own stretch of diagonals), and after the scan copies them, now without a
+// dependency chain, to their final places in a's buffer, which the walks no
+// longer need. The result ends in `a`.
+template <class Team>
+__device__ int team_xor_lds(const Team &t, int *a, int na, const int *b, int nb, int *c) {
+    const int n = na + nb;
+    const int K = (n + Team::size - 1) / Team::size;
+    const int l = t.lane();
+    const int d0 = l * K < n ? l * K : n;
+    const int d1 = d0 + K < n ? d0 + K : n;
+    const int cnt = mp_walk<true>(a, na, b, nb, d0, d1, c + d0);
+    int total;
+    const int ex = t.scan(cnt, total);  // every lane's walk is over after this
+    t.sync();
+    for (int q = 0; q < cnt; q++) a[ex + q] = c[d0 + q];
     return total;
 }
 
@@ -877,11 +898,11 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
             if (gl == 0) d.ess[lj] = 0;
             for (int i = gl; i < na; i += G) sA[i] = d.pool[aoff + i];
             t.sync();
-            nc = team_xor(t, sA, na, sB, clen, sC);
+            nc = team_xor_lds(t, sA, na, sB, clen, sC);
             t.sync();
-            int *tmp = sB;
-            sB = sC;
-            sC = tmp;
+            int *tmp = sB;  // the sum is in sA's buffer
+            sB = sA;
+            sA = tmp;
             nl = nc ? sB[nc - 1] : -1;
         } else if (HANDOFF) {
             handoff = true;
@@ -1004,19 +1025,20 @@ __global__ void __launch_bounds__(kBlock) k_phase2_group(Dev d) {
     }
 }
 
-// Columns handed off by the lane groups: one block (256 lanes) per column,
-// 4096-entry LDS buffers, merging straight in the pool beyond that.
+// Columns handed off by the lane groups: one block of kLongBlock lanes per
+// column, 4096-entry LDS buffers, merging straight in the pool beyond that.
 constexpr int kLongW = 4096;
+constexpr int kLongBlock = 1024;
 
-__global__ void __launch_bounds__(kBlock) k_phase2_long(Dev d) {
+__global__ void __launch_bounds__(kLongBlock) k_phase2_long(Dev d) {
     __shared__ int lds[3 * kLongW];
-    __shared__ int s_scan[kBlock / 64];
+    __shared__ int s_scan[kLongBlock / 64];
     __shared__ unsigned long long s_word;
-    const BlockTeam t{s_scan, &s_word};
+    const BlockTeam<kLongBlock> t{s_scan, &s_word};
     const int nlong = d.ctrl->nlong;
     Phase2Counters ctr;
     for (int w = blockIdx.x; w < nlong; w += gridDim.x)
-        phase2_column<BlockTeam, kLongW, false>(t, d, d.longw[w], lds, ctr);
+        phase2_column<BlockTeam<kLongBlock>, kLongW, false>(t, d, d.longw[w], lds, ctr);
     if (threadIdx.x == 0) {
         if (ctr.ovf) d.ctrl->overflow = 1;
         if (ctr.nadd) {
@@ -1082,7 +1104,7 @@ void launch_phase2(const Dev &d, int lanes, int wlds, int grid, int long_grid, h
         return;
     }
     k<<<grid, kBlock, 0, s>>>(d);
-    k_phase2_long<<<long_grid, kBlock, 0, s>>>(d);
+    k_phase2_long<<<long_grid, kLongBlock, 0, s>>>(d);
 }
 
 // After Phase II: publish new pivots, clear_positives, ess_hat
