@@ -78,6 +78,8 @@ struct Context {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> iter_events;
     std::vector<hipEvent_t> p2_events;  // start/end pairs of Phase II launches
+    Ctrl *h_ctrl = nullptr;                // pinned mirror of d.ctrl
+    unsigned long long *h_word = nullptr;  // pinned scratch words
     bool init = false;
 
     Context() = default;
@@ -92,6 +94,8 @@ struct Context {
                         h_rows, h_cols, h_low, h_ess, b_moff, b_noff, b_rows, b_cols, b_err};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
+        if (h_ctrl) (void)hipHostFree(h_ctrl);
+        if (h_word) (void)hipHostFree(h_word);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         for (hipEvent_t e : iter_events) (void)hipEventDestroy(e);

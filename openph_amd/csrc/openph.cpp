@@ -117,9 +117,10 @@ static int err_from_bits(int bits) {
 static int compact(Context &c, const RunCfg &cfg, hipStream_t s, int *ncompact, size_t &lcap) {
     Dev &d = c.d;
     launch_compact_scan(d, s);
-    unsigned long long live = 0;
-    OPH_CHECK(hipMemcpyAsync(&live, &d.ctrl->pool_top, sizeof(live), hipMemcpyDeviceToHost, s));
+    OPH_CHECK(hipMemcpyAsync(&c.h_word[0], &d.ctrl->pool_top, sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, s));
     OPH_CHECK(hipStreamSynchronize(s));
+    const unsigned long long live = c.h_word[0];
     size_t cap = lcap;
     size_t want = cap;
     if (live * 2 > cap) want = std::max<size_t>(cap * 2, live * 2);
@@ -130,8 +131,10 @@ static int compact(Context &c, const RunCfg &cfg, hipStream_t s, int *ncompact, 
     std::swap(c.cap_pool, c.cap_pool2);
     // the kernels now use pool2's buffer, logically `want` entries long
     lcap = want;
-    unsigned long long capv = lcap;
-    OPH_CHECK(hipMemcpyAsync(&d.ctrl->pool_cap, &capv, sizeof(capv), hipMemcpyHostToDevice, s));
+    c.h_word[1] = lcap;
+    OPH_CHECK(hipMemcpyAsync(&d.ctrl->pool_cap, &c.h_word[1], sizeof(unsigned long long),
+                             hipMemcpyHostToDevice, s));
+    OPH_CHECK(hipStreamSynchronize(s));  // h_word[1] may be rewritten by the next compaction
     (*ncompact)++;
     return OPENPH_OK;
 }
@@ -177,7 +180,13 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
     size_t lcap = std::min(c.cap_pool, pool_want);
     launch_ctrl_init(d, (unsigned long long)lcap, s);
     launch_setup(d, rows1, cols1, s);
-    Ctrl h{};
+    // the control block is read back into pinned memory: a copy to pageable
+    // memory is staged synchronously by the runtime and serializes host
+    // threads that drive other streams
+    if (!c.h_ctrl) OPH_CHECK(hipHostMalloc((void **)&c.h_ctrl, sizeof(Ctrl), hipHostMallocDefault));
+    if (!c.h_word) OPH_CHECK(hipHostMalloc((void **)&c.h_word, 4 * sizeof(unsigned long long),
+                                           hipHostMallocDefault));
+    Ctrl &h = *c.h_ctrl;
     OPH_CHECK(hipMemcpyAsync(&h, d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
     OPH_CHECK(hipStreamSynchronize(s));
     if ((rc = err_from_bits(h.err))) return rc;
@@ -214,21 +223,32 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
         OPH_CHECK(hipEventRecord(evs[0], s));
     }
     while (true) {
+        // Phase I and the worklist. When no low repeats (after the first
+        // iteration) the matrix is reduced; the marks and clears made in
+        // that pass are then facts about the reduced matrix and change
+        // nothing.
         launch_reset_flags(d, s);
         launch_phase1_detect(d, s);
-        if (iter > 1) {
-            OPH_CHECK(hipMemcpyAsync(&h, d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
-            OPH_CHECK(hipStreamSynchronize(s));
-            if (h.err) return err_from_bits(h.err) ? err_from_bits(h.err) : OPENPH_EINVAL;
-            peak = std::max(peak, h.pool_top);
-            if (!h.any_dup) {
-                converged = true;
-                break;
-            }
-        }
-        if (iter > cfg.max_iters) return OPENPH_EITER;
         launch_phase1_mark(d, s);
         launch_apply1(d, s);
+        OPH_CHECK(hipMemcpyAsync(&h, d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+        OPH_CHECK(hipStreamSynchronize(s));
+        if (h.err) return err_from_bits(h.err) ? err_from_bits(h.err) : OPENPH_EINVAL;
+        peak = std::max(peak, h.pool_top);
+        if (iter > 1 && !h.any_dup) {
+            converged = true;
+            break;
+        }
+        if (iter > cfg.max_iters) return OPENPH_EITER;
+        // Phase II grids sized by the work, so that a small pass does not
+        // hold LDS on every CU while other streams' work waits for it.
+        const int nwork = h.nwork;
+        int g1 = grid, g2 = 0;
+        if (lanes > 1) {
+            const int per_block = kBlock / lanes;
+            g1 = std::min(grid, (nwork + per_block - 1) / per_block);
+            g2 = std::min(c.num_cu, nwork);
+        }
         if (stats) {
             size_t k = 2 * (size_t)(iter - 1);
             while (c.p2_events.size() < k + 2) {
@@ -237,10 +257,10 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
                 c.p2_events.push_back(e);
             }
             OPH_CHECK(hipEventRecord(c.p2_events[k], s));
-            launch_phase2(d, lanes, wlds, grid, c.num_cu, s);
+            if (nwork > 0 || lanes == 1) launch_phase2(d, lanes, wlds, g1, g2, s);
             OPH_CHECK(hipEventRecord(c.p2_events[k + 1], s));
-        } else {
-            launch_phase2(d, lanes, wlds, grid, c.num_cu, s);
+        } else if (nwork > 0 || lanes == 1) {
+            launch_phase2(d, lanes, wlds, g1, g2, s);
         }
         launch_apply2(d, s);
         if (time_track && (size_t)iter <= (size_t)m) {
