@@ -49,11 +49,23 @@ extern "C" {
  *               (src/cuda/algorithms/algorithm_factory.cu:10-148)
  *   col_width   accepted and ignored: columns live in an adaptive device
  *               pool, there is no fixed p = col_width * max_nnz.
- *   h_low_true  1-based true low array used only by the trackers (may be
- *               all zeros, as numerics/benchmark_datasets.m:53 passes).
+ *   h_low_true  1-based true low array (0 = none) used only by the
+ *               trackers; NULL skips the err_* trackers.
  *   outputs     h_low[m], h_ess[m]; err_linf, err_lone, err_redu, err_ess,
  *               time_track: m+1 floats each (index = iteration);
  *               num_iters[0].
+ *   trackers    for iterations i = 0 .. num_iters-1, as
+ *               src/cuda/algorithms/tracker.cu:92-134 defines them (0-based
+ *               low and low_true): err_lone[i] = sum_j |low_j - low_true_j|,
+ *               err_linf[i] = max_j |low_j - low_true_j|, err_redu[i] =
+ *               #{j : classes_j == 0} / m, err_ess[i] = #essential(low_true)
+ *               / #{j : ess_j == 1} (1 when equal), time_track[i] = ms of
+ *               iteration i (time_track[0] = 0). Then every one of the m+1
+ *               entries of err_lone and err_linf is divided by the l1 and
+ *               linf norms of h_low_true, as src/cuda/openph.cu:188-193 does
+ *               (a zero norm gives IEEE inf / nan, as there). The sums are
+ *               exact integers rounded once to float; the reference sums
+ *               floats in cuBLAS's unspecified order.
  *
  * The reference has no error channel. Here errors never exit the process:
  * on failure num_iters[0] is set to the negative status code and h_low is
@@ -76,12 +88,26 @@ typedef struct openph_options {
     int lanes_per_column;    /* Phase II lanes per column: 0 = automatic,
                                 else 1, 4, 8, 16, 32 or 64                   */
     int track;               /* 1 = fill the per-iteration trackers          */
-    int slot_elems;          /* minimum pool segment per column (0 = 8)      */
+    int slot_elems;          /* minimum pool segment per column (0 = 32)     */
     int group_lds;           /* LDS entries per lane-group buffer (0 = the
                                 default of lanes_per_column); the compiled
                                 shapes are {4: 32, 64}, {8: 32, 64, 128},
                                 {16: 64, 128, 256}, {32: 256, 512},
                                 {64: 512, 1024}                              */
+    int schedule;            /* how the host drives the pms iterations:
+                                0 = automatic,
+                                1 = read the device state after Phase I and
+                                    after each iteration (Phase II grids
+                                    sized by the work; used when trackers are
+                                    filled),
+                                2 = queue chunks of iterations on the stream
+                                    and read the state once per chunk,
+                                3 = the same chunks replayed from a cached
+                                    hipGraph                                 */
+    int chunk_iters;         /* iterations per chunk after the first one
+                                (schedules 2 and 3; 0 = 3). The first chunk
+                                is as long as the last reduction of a matrix
+                                of the same size took.                       */
 } openph_options;
 
 /* Statistics of one reduction. */
@@ -100,6 +126,8 @@ typedef struct openph_stats {
     int spills;              /* columns whose additions left LDS             */
     int group_lds;           /* LDS entries per lane-group buffer used       */
     int handoffs;            /* columns continued by the block kernel        */
+    int schedule;            /* schedule used (1, 2 or 3, see openph_options) */
+    int host_reads;          /* control-block reads in the iteration loop    */
 } openph_stats;
 
 /*
@@ -107,7 +135,8 @@ typedef struct openph_stats {
  * m-element outputs (1-based low, 0/1 ess); stats may be NULL.
  * trackers (may be NULL) receives 5 x (m+1) floats laid out as
  * err_linf | err_lone | err_redu | err_ess | time_track and is only written
- * when opt->track is set.
+ * when opt->track is set: time_track always, the four err_* arrays when
+ * low_true1 is given (same rules as openph()).
  */
 int openph_v2(const char *algstr, const int *rows1, const int *cols1, int m, int nnz,
               const int *low_true1, const openph_options *opt, int *low1, int *ess,

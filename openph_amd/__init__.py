@@ -54,7 +54,12 @@ class Options(ctypes.Structure):
         ("track", ctypes.c_int),
         ("slot_elems", ctypes.c_int),
         ("group_lds", ctypes.c_int),
+        ("schedule", ctypes.c_int),
+        ("chunk_iters", ctypes.c_int),
     ]
+
+    # values of ``schedule``
+    AUTO, SYNC, QUEUED, GRAPH = 0, 1, 2, 3
 
 
 class Stats(ctypes.Structure):
@@ -74,6 +79,8 @@ class Stats(ctypes.Structure):
         ("spills", ctypes.c_int),
         ("group_lds", ctypes.c_int),
         ("handoffs", ctypes.c_int),
+        ("schedule", ctypes.c_int),
+        ("host_reads", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -143,14 +150,19 @@ def _i32(x):
 
 def ph(algstr, r, c, m, col_width=1, low_true=None, device: int = 0):
     """The mex gateway's contract (src/matlab/ph/ph.cu:11-92): 1-based COO in,
-    eight outputs back. Errors raise OpenPHError instead of printing."""
+    eight outputs back. Errors raise OpenPHError instead of printing.
+
+    low_true (1-based, 0 = none) feeds the err_* trackers; without it they
+    keep the gateway's initial -1."""
     rows = _i32(r)
     cols = _i32(c)
     m = int(m)
     nnz = rows.shape[0]
     if cols.shape[0] != nnz:
         raise OpenPHError(-1, "rows and cols differ in length")
-    lt = _i32(low_true) if low_true is not None else np.zeros(m, np.int32)
+    lt = _i32(low_true) if low_true is not None else None
+    if lt is not None and lt.shape[0] < m:
+        raise OpenPHError(-1, "low_true has fewer than m entries")
     # outputs initialised as the gateway does (ph.cu:72-78)
     low = np.full(m, -1, np.int32)
     ess = np.full(m, -1, np.int32)
@@ -160,8 +172,9 @@ def ph(algstr, r, c, m, col_width=1, low_true=None, device: int = 0):
     opt.device = device
     opt.track = 1
     st = Stats()
-    rc = lib().openph_v2(algstr.encode(), _iptr(rows), _iptr(cols), m, nnz, _iptr(lt),
-                         ctypes.byref(opt), _iptr(low), _iptr(ess), _fptr(trk), ctypes.byref(st))
+    rc = lib().openph_v2(algstr.encode(), _iptr(rows), _iptr(cols), m, nnz,
+                         _iptr(lt) if lt is not None else None, ctypes.byref(opt), _iptr(low),
+                         _iptr(ess), _fptr(trk), ctypes.byref(st))
     if rc != 0:
         raise OpenPHError(rc, f"ph('{algstr}')")
     return (low, ess, trk[0], trk[1], trk[2], trk[3], trk[4],

@@ -30,6 +30,33 @@ struct Ctrl {
     int spills;                    // Phase II columns that left LDS (cumulative)
     int nlong;                     // columns handed to the block kernel
     int handoffs;                  // the same, cumulative
+    // iteration control on the device, so that several iterations can be
+    // queued (or replayed from a graph) between two reads of this block
+    int halt;                      // kHalt* reason; 0 = keep iterating
+    int iter;                      // current iteration (1 = the first)
+    int max_iters;                 // iteration cap
+    int stalls;                    // consecutive overflowing passes without progress
+    int prev_marks;                // marks at the end of the previous iteration
+    int overflows;                 // Phase II passes cut short by a full pool
+    unsigned long long prev_add;   // additions at the end of the previous iteration
+    unsigned long long pool_peak;  // largest pool_top seen at an iteration end
+    // trackers of one iteration (tracker.cu:92-134), as exact integers
+    unsigned long long trk_l1;     // sum_j |low[j] - low_true[j]|
+    int trk_linf;                  // max_j |low[j] - low_true[j]|
+    int trk_unreduced;             // #{j : classes[j] == 0}
+    long long trk_ess;             // #{j : ess[j] == 1}
+};
+
+// Why the iteration kernels stopped (Ctrl::halt). Every iteration kernel
+// returns at once while halt is set; the host clears it to resume.
+enum : int {
+    kHaltNone = 0,
+    kHaltConverged = 1,  // no low repeats: reduced (is_reduced, pms_functions.cu:7-27)
+    kHaltCompact = 2,    // the pool overflowed or is 3/4 full: compact, then resume
+    kHaltStuck = 3,      // lows repeat but nothing changed: internal fault
+    kHaltIters = 4,      // iteration cap reached
+    kHaltNoMem = 5,      // two overflowing passes in a row made no progress
+    kHaltError = 6,      // err bits set
 };
 
 // Validation error bits (Ctrl::err).
@@ -53,7 +80,7 @@ struct Dev {
     int *newpiv = nullptr, *slot = nullptr, *len = nullptr, *cap = nullptr;
     int *work = nullptr;                     // Phase II worklist
     int *longw = nullptr;                    // columns handed to the block kernel
-    int4 *pivdesc = nullptr;                 // per row: {pivot column, off, len, 0}
+    int4 *pivdesc = nullptr;                 // per row: {pivot column, off, len, left}
     unsigned *off = nullptr, *off2 = nullptr;
     unsigned long long *scan_tmp = nullptr;  // tiled-scan partials
     int *slotmap = nullptr;                  // pseudo-dim + 1 -> slot
@@ -63,12 +90,20 @@ struct Dev {
 };
 
 // ---- launchers (kernels.hip) ----
-void launch_ctrl_init(const Dev &d, unsigned long long cap, hipStream_t s);
+void launch_ctrl_init(const Dev &d, unsigned long long cap, int max_iters, hipStream_t s);
 // COO -> per-column state and pool layout (slot-granular segments).
 void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t s);
 void launch_slots(const Dev &d, hipStream_t s);
 void launch_phase0(const Dev &d, hipStream_t s);
-void launch_reset_flags(const Dev &d, hipStream_t s);
+// Start of an iteration: iter + 1 and the per-iteration flags reset.
+void launch_iter_begin(const Dev &d, hipStream_t s);
+// End of an iteration: progress, stall, overflow and pool-fill checks that
+// set Ctrl::halt.
+void launch_iter_end(const Dev &d, hipStream_t s);
+void launch_set_halt(const Dev &d, int value, hipStream_t s);
+// Device clock (wall_clock64) of the current iteration into
+// stamps[(2 * (iter - 1) + which) % cap], unless halted.
+void launch_stamp(const Dev &d, unsigned long long *stamps, int which, int cap, hipStream_t s);
 void launch_phase1_detect(const Dev &d, hipStream_t s);
 void launch_phase1_mark(const Dev &d, hipStream_t s);
 void launch_apply1(const Dev &d, hipStream_t s);
@@ -87,6 +122,13 @@ int phase2_default_lds(int lanes);
 int phase2_blocks_per_cu(int lanes, int wlds);
 void launch_apply2(const Dev &d, hipStream_t s);
 void launch_final(const Dev &d, int *low1_out, int *ess_out, hipStream_t s);
+// ess_hat_final (kernels.cu:103-111) alone, and update_classes_final
+// (kernels.cu:79-91) on classes[]: the state the reference tracks after its
+// last iteration. Needs k_final_ess first; classes[] is not used afterwards.
+void launch_final_ess(const Dev &d, hipStream_t s);
+void launch_classes_final(const Dev &d, hipStream_t s);
+// The trackers of the current state into Ctrl::trk_* (0-based low_true).
+void launch_track(const Dev &d, const int *low_true0, hipStream_t s);
 // Compaction, step 1: new offsets of the live columns in off2 and the live
 // total in ctrl->pool_top (and scan_tmp[ntiles]).
 void launch_compact_scan(const Dev &d, hipStream_t s);

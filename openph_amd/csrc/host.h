@@ -55,6 +55,21 @@ inline hipError_t grow(T **p, size_t &cap, size_t need) {
     return hipSuccess;
 }
 
+// An instantiated graph of `iters` loop iterations (Phase I, worklist,
+// Phase II on fixed persistent grids, bookkeeping, checks), valid for the
+// device buffers and shape it was captured with.
+struct GraphEntry {
+    bool valid = false;
+    Dev key;
+    int lanes = 0, wlds = 0, grid = 0, long_grid = 0, iters = 0;
+    bool timed = false;
+    unsigned long long used = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+constexpr int kStampRing = 128;  // device-clock stamps kept per chunk (2 per iteration)
+
 // Device workspace of one GPU, reused by every call on that GPU.
 struct Context {
     int device = 0;
@@ -80,6 +95,18 @@ struct Context {
     std::vector<hipEvent_t> p2_events;  // start/end pairs of Phase II launches
     Ctrl *h_ctrl = nullptr;                // pinned mirror of d.ctrl
     unsigned long long *h_word = nullptr;  // pinned scratch words
+    // queued / graph schedules: Phase II events of the iterations of one
+    // chunk, cached graphs, and the iteration count of the last reduction
+    // of each size (the first chunk's length)
+    std::vector<hipEvent_t> chunk_events;
+    GraphEntry graphs[4];
+    unsigned long long graph_clock = 0;
+    int last_m = -1, last_nnz = -1, last_iters = 0;
+    unsigned long long *d_stamps = nullptr, *h_stamps = nullptr;  // graph timing
+    int wall_khz = 0;                                            // device clock rate
+    // trackers: 0-based low_true on the device
+    int *t_low = nullptr;
+    size_t c_tlow = 0;
     bool init = false;
 
     Context() = default;
@@ -91,20 +118,47 @@ struct Context {
         void *bufs[] = {d.low, d.arglow, d.classes, d.ess, d.left, d.beta, d.first, d.clear,
                         d.newpiv, d.slot, d.len, d.cap, d.work, d.longw, d.pivdesc, d.off, d.off2,
                         d.scan_tmp, d.slotmap, d.tile_max, d.tile_pre, d.pool, d.pool2, d.ctrl,
-                        h_rows, h_cols, h_low, h_ess, b_moff, b_noff, b_rows, b_cols, b_err};
+                        h_rows, h_cols, h_low, h_ess, b_moff, b_noff, b_rows, b_cols, b_err,
+                        t_low};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
         if (h_ctrl) (void)hipHostFree(h_ctrl);
         if (h_word) (void)hipHostFree(h_word);
+        if (h_stamps) (void)hipHostFree(h_stamps);
+        if (d_stamps) (void)hipFree(d_stamps);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         for (hipEvent_t e : iter_events) (void)hipEventDestroy(e);
         for (hipEvent_t e : p2_events) (void)hipEventDestroy(e);
+        for (hipEvent_t e : chunk_events) (void)hipEventDestroy(e);
+        for (GraphEntry &g : graphs) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            if (g.graph) (void)hipGraphDestroy(g.graph);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
+// How the host drives the iteration loop.
+enum Schedule : int {
+    kSchedAuto = 0,
+    kSchedSync = 1,    // read the control block twice per iteration; Phase II
+                       // grids sized by the worklist
+    kSchedQueued = 2,  // queue `chunk` iterations between reads; fixed grids
+    kSchedGraph = 3,   // the same chunks replayed from a cached hipGraph
+};
+
+// Per-iteration trackers of the drop-in (tracker.cu:92-134). Arrays of m+1
+// floats, any of which may be null; low_true1 is the caller's 1-based array.
+struct Track {
+    const int *low_true1 = nullptr;
+    float *err_linf = nullptr, *err_lone = nullptr, *err_redu = nullptr, *err_ess = nullptr;
+    float *time_track = nullptr;
+};
+
 struct RunCfg {
+    int schedule = kSchedAuto;
+    int chunk = 0;     // iterations per chunk after the first (0 = 3)
     int slot0 = 32;
     int max_iters = 1000000;
     size_t pool_elems = 0;

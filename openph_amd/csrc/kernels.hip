@@ -9,8 +9,9 @@
 //   left[i]    leftmost column holding row i (INT_MAX if none), beta.cu:2-15
 //   beta[j]    max{i : left[i] = j}, beta.cu:2-15
 // and, new here:
-//   pivdesc[i] {arglow[i], off, len, 0} of the pivot column of row i, so a
-//              column addition finds its operand with one 16-byte load
+//   pivdesc[i] {arglow[i], off, len, left[i]}: the pivot column of row i
+//              with the left rule of row i, so each step of a Phase II
+//              chain finds its operand and its rule with one 16-byte load
 //   work[]     the columns Phase II has to reduce in this iteration
 // Columns live in a pool of int32 row indices: column j occupies
 // pool[off[j] .. off[j]+len[j]) in ascending order inside a segment of
@@ -188,7 +189,7 @@ static void launch_scan(const Dev &d, int mode, unsigned *out, hipStream_t s) {
 // Setup: COO (1-based) -> pool + per-column state
 // ---------------------------------------------------------------------------
 
-__global__ void k_ctrl_init(Ctrl *c, unsigned long long cap) {
+__global__ void k_ctrl_init(Ctrl *c, unsigned long long cap, int max_iters) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         c->pool_top = 0;
         c->pool_cap = cap;
@@ -204,11 +205,19 @@ __global__ void k_ctrl_init(Ctrl *c, unsigned long long cap) {
         c->nlong = 0;
         c->spills = 0;
         c->handoffs = 0;
+        c->halt = kHaltNone;
+        c->iter = 0;
+        c->max_iters = max_iters;
+        c->stalls = 0;
+        c->prev_marks = 0;
+        c->overflows = 0;
+        c->prev_add = 0;
+        c->pool_peak = 0;
     }
 }
 
-void launch_ctrl_init(const Dev &d, unsigned long long cap, hipStream_t s) {
-    k_ctrl_init<<<1, 64, 0, s>>>(d.ctrl, cap);
+void launch_ctrl_init(const Dev &d, unsigned long long cap, int max_iters, hipStream_t s) {
+    k_ctrl_init<<<1, 64, 0, s>>>(d.ctrl, cap, max_iters);
 }
 
 __device__ __forceinline__ int lower_bound_i(const int *a, int n, int key) {
@@ -298,6 +307,7 @@ __global__ void k_beta(Dev d) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.m) return;
     int l = d.left[i];
+    d.pivdesc[i].w = l;
     if (l != kIntMax) atomicMax(&d.beta[l], i);
 }
 
@@ -352,7 +362,7 @@ void launch_slots(const Dev &d, hipStream_t s) {
 // Publishes column j as the pivot of row l.
 __device__ __forceinline__ void publish_pivot(const Dev &d, int l, int j) {
     d.arglow[l] = j;
-    d.pivdesc[l] = make_int4(j, (int)d.off[j], d.len[j], 0);
+    d.pivdesc[l] = make_int4(j, (int)d.off[j], d.len[j], d.left[l]);
 }
 
 // ---------------------------------------------------------------------------
@@ -376,6 +386,7 @@ __global__ void k_phase0(Dev d) {
 // table and construction of the Phase II worklist: columns still unknown
 // whose low has a pivot to their left.
 __global__ void k_apply1(Dev d) {
+    if (d.ctrl->halt) return;  // uniform over the grid: no thread reaches the barrier
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     bool take = false;
     if (t < d.m) {
@@ -396,6 +407,8 @@ __global__ void k_apply1(Dev d) {
     block_append(d.work, &d.ctrl->nwork, t, take);
 }
 
+// Phase 0 and its clears; k_apply1 also resets first[] for Phase I (the
+// worklist it builds is dropped by the first k_iter_begin).
 void launch_phase0(const Dev &d, hipStream_t s) {
     int gm = (d.m + kBlock - 1) / kBlock;
     k_phase0<<<gm, kBlock, 0, s>>>(d);
@@ -407,17 +420,72 @@ void launch_apply1(const Dev &d, hipStream_t s) {
     k_apply1<<<gm, kBlock, 0, s>>>(d);
 }
 
-__global__ void k_reset_flags(Ctrl *c) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        c->any_dup = 0;
-        c->overflow = 0;
-        c->nwork = 0;
-        c->nlong = 0;
-    }
+// ---------------------------------------------------------------------------
+// Iteration control (pms.cu:42-114). The loop's decisions are taken on the
+// device, so that the host can queue several iterations, or replay them from
+// a graph, and read the control block once at the end. Every kernel of an
+// iteration returns at once while Ctrl::halt is set.
+// ---------------------------------------------------------------------------
+
+__global__ void k_iter_begin(Ctrl *c) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || c->halt) return;
+    c->iter += 1;
+    c->any_dup = 0;
+    c->overflow = 0;
+    c->nwork = 0;
+    c->nlong = 0;
 }
 
-void launch_reset_flags(const Dev &d, hipStream_t s) {
-    k_reset_flags<<<1, 64, 0, s>>>(d.ctrl);
+// After Phase II and its bookkeeping. While lows repeat, pms marks a
+// column or adds one in every iteration (the first
+// occurrence of the largest repeated low is marked, or its repeats are
+// reduced), so an iteration without either is a fault unless the pool ran
+// out; two overflowing passes in a row without progress (the second right
+// after a compaction) mean the pool budget cannot hold the work.
+__global__ void k_iter_end(Ctrl *c) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || c->halt) return;
+    if (c->err) {
+        c->halt = kHaltError;
+        return;
+    }
+    const bool progress = c->additions != c->prev_add || c->marks != c->prev_marks;
+    if (c->pool_top > c->pool_peak) c->pool_peak = c->pool_top;
+    if (!c->overflow && !progress && c->any_dup) {
+        c->halt = kHaltStuck;
+        return;
+    }
+    c->stalls = (c->overflow && !progress) ? c->stalls + 1 : 0;
+    if (c->stalls >= 2) {
+        c->halt = kHaltNoMem;
+        return;
+    }
+    c->prev_add = c->additions;
+    c->prev_marks = c->marks;
+    if (c->overflow) c->overflows += 1;
+    if (c->overflow || c->pool_top * 4 > c->pool_cap * 3) c->halt = kHaltCompact;
+}
+
+__global__ void k_set_halt(Ctrl *c, int value) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) c->halt = value;
+}
+
+// Device clock at a point of the current iteration (which = 0 before
+// Phase II, 1 after it) into a ring of `cap` entries; used to time Phase II
+// inside graphs, which this runtime does not let record events once another
+// library in the process has initialised it.
+__global__ void k_stamp(Ctrl *c, unsigned long long *stamps, int which, int cap) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || c->halt) return;
+    stamps[(2 * (c->iter - 1) + which) % cap] = (unsigned long long)wall_clock64();
+}
+
+void launch_stamp(const Dev &d, unsigned long long *stamps, int which, int cap, hipStream_t s) {
+    k_stamp<<<1, 64, 0, s>>>(d.ctrl, stamps, which, cap);
+}
+
+void launch_iter_begin(const Dev &d, hipStream_t s) { k_iter_begin<<<1, 64, 0, s>>>(d.ctrl); }
+void launch_iter_end(const Dev &d, hipStream_t s) { k_iter_end<<<1, 64, 0, s>>>(d.ctrl); }
+void launch_set_halt(const Dev &d, int value, hipStream_t s) {
+    k_set_halt<<<1, 64, 0, s>>>(d.ctrl, value);
 }
 
 // ---------------------------------------------------------------------------
@@ -447,6 +515,7 @@ constexpr int kHash = 2 * kTile;  // open addressing, load factor <= 1/2
 __global__ void __launch_bounds__(kBlock) k_first(Dev d) {
     __shared__ int hkey[kHash];
     __shared__ int hval[kHash];
+    if (d.ctrl->halt) return;
     for (int i = threadIdx.x; i < kHash; i += kBlock) {
         hkey[i] = -1;
         hval[i] = kIntMax;
@@ -497,6 +566,7 @@ __device__ __forceinline__ void dup_state(const Dev &d, int j, int lj, int &dup,
 __global__ void __launch_bounds__(kBlock) k_tile(Dev d) {
     __shared__ int smax[kMaxSlots];
     __shared__ int sdup;
+    if (d.ctrl->halt) return;
     if (threadIdx.x < kMaxSlots) smax[threadIdx.x] = -1;
     if (threadIdx.x == 0) sdup = 0;
     __syncthreads();
@@ -520,9 +590,26 @@ __global__ void __launch_bounds__(kBlock) k_tile(Dev d) {
     if (threadIdx.x == 0 && sdup) atomicOr(&d.ctrl->any_dup, 1);
 }
 
-// Exclusive max-scan of the tile maxima over tiles, per slot. One block.
+// Convergence test (is_reduced, pms_functions.cu:7-27: no non-negative low
+// occurs twice, from k_tile's flag; the first iteration always runs, as in
+// pms.cu:39-44), the iteration cap, and the exclusive max-scan of the tile
+// maxima over tiles, per slot. One block.
 __global__ void __launch_bounds__(1024) k_tilescan(Dev d) {
     __shared__ int lds[16];
+    __shared__ int s_halt;
+    if (threadIdx.x == 0) {
+        Ctrl *c = d.ctrl;
+        int h = c->halt;
+        if (!h) {
+            if (c->err) h = kHaltError;
+            else if (c->iter > 1 && !c->any_dup) h = kHaltConverged;
+            else if (c->iter > c->max_iters) h = kHaltIters;
+            c->halt = h;
+        }
+        s_halt = h;
+    }
+    __syncthreads();
+    if (s_halt) return;
     const int nslots = d.ctrl->nslots;
     for (int s = 0; s < nslots && s < kMaxSlots; s++) {
         int running = -1;
@@ -540,6 +627,7 @@ __global__ void __launch_bounds__(1024) k_tilescan(Dev d) {
 __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
     __shared__ int lds[kBlock / 64];
     __shared__ int exl[kMaxSlots][kBlock];
+    if (d.ctrl->halt) return;
     const int nslots = d.ctrl->nslots;
     const int base = blockIdx.x * kTile + threadIdx.x * kItems;
     int lo[kItems], sl[kItems], dup[kItems], fst[kItems];
@@ -632,6 +720,7 @@ __device__ __forceinline__ int merge_xor(const int *__restrict__ a, int na,
 // One lane per column over all m columns, every addition into a fresh pool
 // segment. Kept as the lanes = 1 configuration of the autotuner.
 __global__ void __launch_bounds__(kBlock) k_phase2_lane(Dev d) {
+    if (d.ctrl->halt) return;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     bool active = false;
     int lj = -1, p = -1;
@@ -938,11 +1027,11 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
         lj = nl;
         if (lj < 0) break;
         if (gl == 0) d.clear[lj] = 1;
-        if (d.left[lj] == j) {
+        pd = d.pivdesc[lj];  // .w = left[lj]: one load for the rule and the pivot
+        if (pd.w == j) {
             becomes_pivot = true;
             break;
         }
-        pd = d.pivdesc[lj];
     }
     bool committed = false;
     if (!lost && dirty) {
@@ -1003,6 +1092,7 @@ __global__ void __launch_bounds__(kBlock) k_phase2_group(Dev d) {
     const WaveTeam<G> t{(int)(threadIdx.x % G), (int)((threadIdx.x & 63) / G)};
     const int gid = threadIdx.x / G;
     int *const buf = lds + gid * 3 * W;
+    if (d.ctrl->halt) return;
     const int nwork = d.ctrl->nwork;
     Phase2Counters ctr;
     for (int w = blockIdx.x * NG + gid; w < nwork; w += gridDim.x * NG)
@@ -1035,6 +1125,7 @@ __global__ void __launch_bounds__(kLongBlock) k_phase2_long(Dev d) {
     __shared__ int s_scan[kLongBlock / 64];
     __shared__ unsigned long long s_word;
     const BlockTeam<kLongBlock> t{s_scan, &s_word};
+    if (d.ctrl->halt) return;
     const int nlong = d.ctrl->nlong;
     Phase2Counters ctr;
     for (int w = blockIdx.x; w < nlong; w += gridDim.x)
@@ -1112,7 +1203,7 @@ void launch_phase2(const Dev &d, int lanes, int wlds, int grid, int long_grid, h
 // touches its own column first, so one pass suffices.
 __global__ void k_apply2(Dev d) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= d.m) return;
+    if (t >= d.m || d.ctrl->halt) return;
     int np = d.newpiv[t];
     if (np > -1) {
         publish_pivot(d, t, np);
@@ -1168,6 +1259,95 @@ void launch_final(const Dev &d, int *low1_out, int *ess_out, hipStream_t s) {
     k_output<<<gm, kBlock, 0, s>>>(d, low1_out, ess_out);
 }
 
+void launch_final_ess(const Dev &d, hipStream_t s) {
+    if (d.m) k_final_ess<<<(d.m + kBlock - 1) / kBlock, kBlock, 0, s>>>(d);
+}
+
+// update_classes_final (kernels.cu:79-91). The stores cannot conflict: a
+// column with a low is never itself a low (it is a negative, and lows are
+// positives), and ess_hat_final has zeroed ess of every low.
+__global__ void k_classes_final(Dev d) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.m) return;
+    int lt = d.low[t];
+    if (lt > -1) {
+        d.classes[t] = -1;
+        d.classes[lt] = 1;
+    } else if (d.ess[t] == 1) {
+        d.classes[t] = 2;
+    }
+}
+
+void launch_classes_final(const Dev &d, hipStream_t s) {
+    if (d.m) k_classes_final<<<(d.m + kBlock - 1) / kBlock, kBlock, 0, s>>>(d);
+}
+
+// ---------------------------------------------------------------------------
+// Trackers (tracker.cu:92-134) as exact integer reductions: the l1 and linf
+// distance of low to low_true (both 0-based, as the reference compares them
+// after indexShiftDown, openph.cu:113), the unreduced count #{classes == 0}
+// and the essential estimate #{ess == 1}. One 64-bit atomic per block and
+// quantity; the host turns them into the reference's floats.
+// ---------------------------------------------------------------------------
+
+__global__ void k_track_zero(Ctrl *c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c->trk_l1 = 0;
+        c->trk_linf = 0;
+        c->trk_unreduced = 0;
+        c->trk_ess = 0;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_track(Dev d, const int *__restrict__ low_true0) {
+    __shared__ unsigned long long s_l1[kBlock / 64];
+    __shared__ int s_inf[kBlock / 64], s_un[kBlock / 64], s_es[kBlock / 64];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long l1 = 0;
+    int inf = 0, un = 0, es = 0;
+    if (t < d.m) {
+        const long long diff = (long long)d.low[t] - (long long)low_true0[t];
+        const int a = (int)(diff < 0 ? -diff : diff);
+        l1 = (unsigned long long)a;
+        inf = a;
+        un = d.classes[t] == 0;
+        es = d.ess[t] == 1;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        l1 += __shfl_down(l1, o, 64);
+        const int x = __shfl_down(inf, o, 64);
+        inf = x > inf ? x : inf;
+        un += __shfl_down(un, o, 64);
+        es += __shfl_down(es, o, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        s_l1[w] = l1;
+        s_inf[w] = inf;
+        s_un[w] = un;
+        s_es[w] = es;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; k++) {
+            l1 += s_l1[k];
+            inf = s_inf[k] > inf ? s_inf[k] : inf;
+            un += s_un[k];
+            es += s_es[k];
+        }
+        if (l1) atomicAdd(&d.ctrl->trk_l1, l1);
+        if (inf) atomicMax(&d.ctrl->trk_linf, inf);
+        if (un) atomicAdd(&d.ctrl->trk_unreduced, un);
+        if (es) atomicAdd((unsigned long long *)&d.ctrl->trk_ess, (unsigned long long)es);
+    }
+}
+
+void launch_track(const Dev &d, const int *low_true0, hipStream_t s) {
+    k_track_zero<<<1, 64, 0, s>>>(d.ctrl);
+    if (d.m) k_track<<<(d.m + kBlock - 1) / kBlock, kBlock, 0, s>>>(d, low_true0);
+}
+
 // ---------------------------------------------------------------------------
 // Pool compaction: live columns are copied, in column order, to the other
 // pool; cleared columns are dropped (never reduced again, never a pivot).
@@ -1190,7 +1370,7 @@ __global__ void k_copy_cols(Dev d) {
     d.cap[j] = (int)sz;
     if (d.classes[j] == -1) {
         int l = d.low[j];
-        if (l > -1 && d.arglow[l] == j) d.pivdesc[l] = make_int4(j, (int)no, n, 0);
+        if (l > -1 && d.arglow[l] == j) d.pivdesc[l] = make_int4(j, (int)no, n, d.left[l]);
     }
 }
 

@@ -94,6 +94,8 @@ RunCfg cfg_from(const openph_options *opt, int nnz) {
         r.lanes = opt->lanes_per_column;
         r.wlds = opt->group_lds;
         r.track = opt->track != 0;
+        r.schedule = opt->schedule;
+        r.chunk = opt->chunk_iters;
         if (opt->slot_elems > 0) r.slot0 = std::min(opt->slot_elems, 1 << 16);
     }
     if (r.pool_elems > r.pool_max) r.pool_elems = r.pool_max;
@@ -139,19 +141,175 @@ static int compact(Context &c, const RunCfg &cfg, hipStream_t s, int *ncompact, 
     return OPENPH_OK;
 }
 
+// Reads the control block into pinned memory and waits for it.
+static int read_ctrl(Context &c, hipStream_t s) {
+    OPH_CHECK(hipMemcpyAsync(c.h_ctrl, c.d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+    OPH_CHECK(hipStreamSynchronize(s));
+    return OPENPH_OK;
+}
+
+static int halt_status(const Ctrl &h) {
+    switch (h.halt) {
+        case kHaltStuck: return OPENPH_EITER;
+        case kHaltIters: return OPENPH_EITER;
+        case kHaltNoMem: return OPENPH_ENOMEM;
+        case kHaltError: {
+            int rc = err_from_bits(h.err);
+            return rc ? rc : OPENPH_EINVAL;
+        }
+        default: return OPENPH_OK;
+    }
+}
+
+static int grow_events(std::vector<hipEvent_t> &v, size_t n) {
+    while (v.size() < n) {
+        hipEvent_t e;
+        OPH_CHECK(hipEventCreate(&e));
+        v.push_back(e);
+    }
+    return OPENPH_OK;
+}
+
+// The Phase II launch shape of one reduction.
+struct Shape {
+    int lanes, wlds, grid, long_grid;
+};
+
+// One loop iteration as queued on a stream (pms.cu:44-114): Phase I and the
+// worklist, Phase II on fixed persistent grids (the kernels walk the
+// worklist, whatever its length), the bookkeeping, and the device-side
+// checks that stop the remaining queued iterations (Ctrl::halt). Phase II
+// is bracketed by the events e0/e1 or, inside graphs, by device-clock
+// stamps, when either is given.
+static int enqueue_iteration(const Dev &d, const Shape &sh, hipStream_t s, hipEvent_t e0,
+                             hipEvent_t e1, unsigned long long *stamps) {
+    launch_iter_begin(d, s);
+    launch_phase1_detect(d, s);
+    launch_phase1_mark(d, s);
+    launch_apply1(d, s);
+    if (e0) OPH_CHECK(hipEventRecord(e0, s));
+    if (stamps) launch_stamp(d, stamps, 0, kStampRing, s);
+    launch_phase2(d, sh.lanes, sh.wlds, sh.grid, sh.long_grid, s);
+    if (stamps) launch_stamp(d, stamps, 1, kStampRing, s);
+    if (e1) OPH_CHECK(hipEventRecord(e1, s));
+    launch_apply2(d, s);
+    launch_iter_end(d, s);
+    return OPENPH_OK;
+}
+
+// OPENPH_DEBUG=1 in the environment reports recoverable runtime refusals
+// (a graph capture that falls back to queued launches) on stderr.
+static void debug_note(const char *what, hipError_t e) {
+    static const bool on = std::getenv("OPENPH_DEBUG") != nullptr;
+    if (on) std::fprintf(stderr, "openph: %s: %s\n", what, hipGetErrorString(e));
+}
+
+static void drop_graph(GraphEntry &g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g = GraphEntry();
+}
+
+// The cached graph of `iters` iterations for the current buffers and shape,
+// captured on the context's own stream (the caller's may be the null
+// stream, which cannot be captured) when there is none. A timed graph
+// brackets Phase II with device-clock stamps: event records inside a
+// capture are refused by this runtime once PyTorch has initialised it.
+static int graph_for(Context &c, const Shape &sh, int iters, bool timed, GraphEntry **out) {
+    const Dev &d = c.d;
+    GraphEntry *hit = nullptr, *victim = &c.graphs[0];
+    for (GraphEntry &g : c.graphs) {
+        if (g.valid && std::memcmp(&g.key, &d, sizeof(Dev)) == 0 && g.lanes == sh.lanes &&
+            g.wlds == sh.wlds && g.grid == sh.grid && g.long_grid == sh.long_grid &&
+            g.iters == iters && g.timed == timed) {
+            hit = &g;
+            break;
+        }
+        if (!g.valid || g.used < victim->used) victim = &g;
+    }
+    if (!hit) {
+        drop_graph(*victim);
+        if (!c.stream) OPH_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        if (timed && !c.d_stamps) {
+            OPH_CHECK(hipMalloc((void **)&c.d_stamps, kStampRing * sizeof(unsigned long long)));
+            OPH_CHECK(hipHostMalloc((void **)&c.h_stamps, kStampRing * sizeof(unsigned long long),
+                                    hipHostMallocDefault));
+            OPH_CHECK(hipDeviceGetAttribute(&c.wall_khz, hipDeviceAttributeWallClockRate,
+                                            c.device));
+        }
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed);
+        if (e != hipSuccess) {
+            debug_note("hipStreamBeginCapture", e);
+            last_hip_error() = e;
+            return OPENPH_EDEVICE;
+        }
+        int rc = OPENPH_OK;
+        for (int k = 0; k < iters && !rc; k++)
+            rc = enqueue_iteration(d, sh, c.stream, nullptr, nullptr,
+                                   timed ? c.d_stamps : nullptr);
+        if (rc) debug_note("capture of one iteration", last_hip_error());
+        e = hipStreamEndCapture(c.stream, &graph);
+        if (rc || e != hipSuccess) {
+            if (e != hipSuccess) debug_note("hipStreamEndCapture", e);
+            if (graph) (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) last_hip_error() = e;
+            return rc ? rc : OPENPH_EDEVICE;
+        }
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            debug_note("hipGraphInstantiate", e);
+            (void)hipGraphDestroy(graph);
+            last_hip_error() = e;
+            return OPENPH_EDEVICE;
+        }
+        victim->valid = true;
+        victim->key = d;
+        victim->lanes = sh.lanes;
+        victim->wlds = sh.wlds;
+        victim->grid = sh.grid;
+        victim->long_grid = sh.long_grid;
+        victim->iters = iters;
+        victim->timed = timed;
+        victim->graph = graph;
+        victim->exec = exec;
+        hit = victim;
+    }
+    hit->used = ++c.graph_clock;
+    *out = hit;
+    return OPENPH_OK;
+}
+
+// Trackers of the current state into slot `it` (tracker.cu:92-134); reads
+// the control block in any case.
+static int track_state(Context &c, const Track *tr, int it, hipStream_t s, int m,
+                       long long ess_true) {
+    const bool slot = tr && tr->low_true1 && it <= m;
+    if (slot) launch_track(c.d, c.t_low, s);
+    int rc = read_ctrl(c, s);
+    if (rc || !slot) return rc;
+    const Ctrl &h = *c.h_ctrl;
+    if (tr->err_lone) tr->err_lone[it] = (float)h.trk_l1;
+    if (tr->err_linf) tr->err_linf[it] = (float)h.trk_linf;
+    if (tr->err_redu) tr->err_redu[it] = (float)h.trk_unreduced / (float)m;
+    if (tr->err_ess)
+        tr->err_ess[it] = h.trk_ess == ess_true ? 1.0f : (float)ess_true / (float)h.trk_ess;
+    return OPENPH_OK;
+}
+
 // The pms driver (pms.cu:1-118) on device-resident 1-based COO.
 static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int *low1,
                    int *ess, hipStream_t s, const RunCfg &cfg, openph_stats *stats,
-                   float *time_track) {
+                   const Track *tr) {
     if (m < 0 || nnz < 0) return OPENPH_EINVAL;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
     if (m == 0) {
-        if (stats) {
-            std::memset(stats, 0, sizeof(*stats));
-            stats->num_iters = 1;
-        }
+        if (stats) stats->num_iters = 1;
         return OPENPH_OK;
     }
-    if (cfg.lanes < 0) return OPENPH_EINVAL;
+    if (cfg.lanes < 0 || cfg.schedule < kSchedAuto || cfg.schedule > kSchedGraph || cfg.chunk < 0)
+        return OPENPH_EINVAL;
     // the initial layout gives every column a segment of >= slot0 entries
     const size_t layout = (size_t)nnz + (size_t)m * (size_t)cfg.slot0;
     if (layout > cfg.pool_max) return OPENPH_ENOMEM;
@@ -175,158 +333,245 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
         OPH_CHECK(hipEventCreate(&c.ev0));
         OPH_CHECK(hipEventCreate(&c.ev1));
     }
-    // logical pool size: the requested one, even when the context holds a
-    // larger buffer from an earlier call
-    size_t lcap = std::min(c.cap_pool, pool_want);
-    launch_ctrl_init(d, (unsigned long long)lcap, s);
-    launch_setup(d, rows1, cols1, s);
     // the control block is read back into pinned memory: a copy to pageable
     // memory is staged synchronously by the runtime and serializes host
     // threads that drive other streams
     if (!c.h_ctrl) OPH_CHECK(hipHostMalloc((void **)&c.h_ctrl, sizeof(Ctrl), hipHostMallocDefault));
     if (!c.h_word) OPH_CHECK(hipHostMalloc((void **)&c.h_word, 4 * sizeof(unsigned long long),
                                            hipHostMallocDefault));
-    Ctrl &h = *c.h_ctrl;
-    OPH_CHECK(hipMemcpyAsync(&h, d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
-    OPH_CHECK(hipStreamSynchronize(s));
-    if ((rc = err_from_bits(h.err))) return rc;
+    // trackers: low_true (1-based, 0 = none) checked, shifted down and
+    // uploaded; its essential count (compute_ess_true, kernels.cu:54-62) and
+    // norms (openph.cu:106-111) are taken here
+    const bool tracking = tr && tr->low_true1;
+    long long ess_true = 0;
+    float norm1 = 0.f, norminf = 0.f;
+    if (tracking) {
+        std::vector<int> lt0((size_t)m);
+        std::vector<char> paired((size_t)m, 0);
+        long long l1 = 0;
+        int linf = 0;
+        for (int j = 0; j < m; j++) {
+            const int v = tr->low_true1[j];
+            if (v < 0 || v > m) return OPENPH_EINVAL;
+            lt0[j] = v - 1;
+            l1 += v;
+            linf = std::max(linf, v);
+            if (v > 0) {
+                paired[j] = 1;
+                paired[v - 1] = 1;
+            }
+        }
+        for (int j = 0; j < m; j++) ess_true += !paired[j];
+        norm1 = (float)l1;
+        norminf = (float)linf;
+        OPH_CHECK(grow(&c.t_low, c.c_tlow, (size_t)m + 1));
+        OPH_CHECK(hipMemcpyAsync(c.t_low, lt0.data(), sizeof(int) * m, hipMemcpyHostToDevice, s));
+    }
+    // logical pool size: the requested one, even when the context holds a
+    // larger buffer from an earlier call
+    size_t lcap = std::min(c.cap_pool, pool_want);
+    launch_ctrl_init(d, (unsigned long long)lcap, cfg.max_iters, s);
+    launch_setup(d, rows1, cols1, s);
+    rc = read_ctrl(c, s);  // also completes the low_true upload from pageable memory
+    if (rc) return rc;
+    if ((rc = err_from_bits(c.h_ctrl->err))) return rc;
+    const int maxlen = c.h_ctrl->maxlen;
+    if (tracking && (rc = track_state(c, tr, 0, s, m, ess_true))) return rc;
     launch_slots(d, s);
     // Phase II lanes per column: short input columns (simplicial complexes
     // of small dimension) take 8 lanes with 128-entry LDS buffers, longer
     // ones (cubical complexes) a full wave with 1024-entry buffers.
-    int lanes = cfg.lanes, wlds = cfg.wlds;
-    if (!lanes) {
-        lanes = h.maxlen <= 4 ? 8 : 64;
-        wlds = h.maxlen <= 4 ? 128 : 1024;
+    Shape sh{cfg.lanes, cfg.wlds, 0, 0};
+    if (!sh.lanes) {
+        sh.lanes = maxlen <= 4 ? 8 : 64;
+        sh.wlds = maxlen <= 4 ? 128 : 1024;
     }
-    const int grid = lanes > 1 ? phase2_blocks_per_cu(lanes, wlds) * c.num_cu : 0;
+    if (sh.lanes > 1) {
+        sh.grid = phase2_blocks_per_cu(sh.lanes, sh.wlds) * c.num_cu;
+        sh.long_grid = c.num_cu;
+    }
+    // the schedule: per-iteration reads when the trackers or the iteration
+    // times are wanted, else queued chunks
+    int sched = cfg.schedule;
+    const bool per_iter = tracking || (tr && tr->time_track);
+    if (per_iter || sched == kSchedAuto) sched = per_iter ? kSchedSync : kSchedQueued;
+    const bool timed = stats != nullptr;
+
     OPH_CHECK(hipEventRecord(c.ev0, s));
     launch_phase0(d, s);
-
-    int ncompact = 0, noverflow = 0;
-    unsigned long long peak = nnz;
-    unsigned long long prev_add = 0;
-    int prev_marks = 0;
-    int stalls = 0;
-    int iter = 1;
-    bool converged = false;
-    if (time_track && c.iter_events.size() < 2) {
-        // grown lazily below
-    }
+    // iteration i is timed from evs[2i] to evs[2i+1] (tic/toc around the
+    // loop body, pms.cu:46-103; the trackers are outside)
     std::vector<hipEvent_t> &evs = c.iter_events;
-    if (time_track) {
-        if (evs.empty()) {
-            hipEvent_t e;
-            OPH_CHECK(hipEventCreate(&e));
-            evs.push_back(e);
-        }
-        OPH_CHECK(hipEventRecord(evs[0], s));
-    }
-    while (true) {
-        // Phase I and the worklist. When no low repeats (after the first
-        // iteration) the matrix is reduced; the marks and clears made in
-        // that pass are then facts about the reduced matrix and change
-        // nothing.
-        launch_reset_flags(d, s);
-        launch_phase1_detect(d, s);
-        launch_phase1_mark(d, s);
-        launch_apply1(d, s);
-        OPH_CHECK(hipMemcpyAsync(&h, d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
-        OPH_CHECK(hipStreamSynchronize(s));
-        if (h.err) return err_from_bits(h.err) ? err_from_bits(h.err) : OPENPH_EINVAL;
-        peak = std::max(peak, h.pool_top);
-        if (iter > 1 && !h.any_dup) {
-            converged = true;
-            break;
-        }
-        if (iter > cfg.max_iters) return OPENPH_EITER;
-        // Phase II grids sized by the work, so that a small pass does not
-        // hold LDS on every CU while other streams' work waits for it.
-        const int nwork = h.nwork;
-        int g1 = grid, g2 = 0;
-        if (lanes > 1) {
-            const int per_block = kBlock / lanes;
-            g1 = std::min(grid, (nwork + per_block - 1) / per_block);
-            g2 = std::min(c.num_cu, nwork);
-        }
-        if (stats) {
-            size_t k = 2 * (size_t)(iter - 1);
-            while (c.p2_events.size() < k + 2) {
-                hipEvent_t e;
-                OPH_CHECK(hipEventCreate(&e));
-                c.p2_events.push_back(e);
+    const bool iter_times = tr && tr->time_track;
+    int ncompact = 0, reads = 0, p2_launches = 0;
+    float p2_ms = 0.f;
+    const Ctrl &h = *c.h_ctrl;
+    if (sched == kSchedSync) {
+        while (true) {
+            const int next = h.iter + 1;
+            if (iter_times && next <= m) {
+                if ((rc = grow_events(evs, 2 * (size_t)next + 2))) return rc;
+                OPH_CHECK(hipEventRecord(evs[2 * next], s));
             }
-            OPH_CHECK(hipEventRecord(c.p2_events[k], s));
-            if (nwork > 0 || lanes == 1) launch_phase2(d, lanes, wlds, g1, g2, s);
-            OPH_CHECK(hipEventRecord(c.p2_events[k + 1], s));
-        } else if (nwork > 0 || lanes == 1) {
-            launch_phase2(d, lanes, wlds, g1, g2, s);
-        }
-        launch_apply2(d, s);
-        if (time_track && (size_t)iter <= (size_t)m) {
-            while (evs.size() <= (size_t)iter) {
-                hipEvent_t e;
-                OPH_CHECK(hipEventCreate(&e));
-                evs.push_back(e);
+            launch_iter_begin(d, s);
+            launch_phase1_detect(d, s);
+            launch_phase1_mark(d, s);
+            launch_apply1(d, s);
+            if ((rc = read_ctrl(c, s))) return rc;
+            reads++;
+            if (h.halt == kHaltConverged) break;
+            if (h.halt) return halt_status(h);
+            const int iter = h.iter;
+            // Phase II grids sized by the work, so that a small pass does not
+            // hold LDS on every CU while other streams' work waits for it
+            Shape g = sh;
+            const int nwork = h.nwork;
+            if (sh.lanes > 1) {
+                const int per_block = kBlock / sh.lanes;
+                g.grid = std::min(sh.grid, (nwork + per_block - 1) / per_block);
+                g.long_grid = std::min(c.num_cu, nwork);
             }
-            OPH_CHECK(hipEventRecord(evs[iter], s));
+            const bool run2 = nwork > 0 || sh.lanes == 1;
+            if (timed) {
+                if ((rc = grow_events(c.p2_events, 2 * (size_t)iter))) return rc;
+                OPH_CHECK(hipEventRecord(c.p2_events[2 * (iter - 1)], s));
+                if (run2) launch_phase2(d, g.lanes, g.wlds, g.grid, g.long_grid, s);
+                OPH_CHECK(hipEventRecord(c.p2_events[2 * (iter - 1) + 1], s));
+            } else if (run2) {
+                launch_phase2(d, g.lanes, g.wlds, g.grid, g.long_grid, s);
+            }
+            p2_launches++;
+            launch_apply2(d, s);
+            launch_iter_end(d, s);
+            if (iter_times && iter <= m) OPH_CHECK(hipEventRecord(evs[2 * iter + 1], s));
+            if (tracking) {
+                if ((rc = track_state(c, tr, iter, s, m, ess_true))) return rc;
+            } else if ((rc = read_ctrl(c, s))) {
+                return rc;
+            }
+            reads++;
+            if (h.halt == kHaltCompact) {
+                if ((rc = compact(c, cfg, s, &ncompact, lcap))) return rc;
+                launch_set_halt(d, kHaltNone, s);
+            } else if (h.halt) {
+                return halt_status(h);
+            }
         }
-        OPH_CHECK(hipMemcpyAsync(&h, d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
-        OPH_CHECK(hipStreamSynchronize(s));
-        if (h.err) return OPENPH_EINVAL;
-        peak = std::max(peak, h.pool_top);
-        // pms makes progress every iteration while lows repeat (the first
-        // occurrence of the largest repeated low is marked, or its repeats
-        // are reduced); no progress without an overflow means a fault.
-        bool progress = h.additions != prev_add || h.marks != prev_marks;
-        if (!h.overflow && !progress && h.any_dup) return OPENPH_EITER;
-        // a full pool with no progress twice in a row (the second time right
-        // after a compaction) means the pool budget cannot hold the work
-        stalls = (h.overflow && !progress) ? stalls + 1 : 0;
-        if (stalls >= 2) return OPENPH_ENOMEM;
-        prev_add = h.additions;
-        prev_marks = h.marks;
-        if (h.overflow) noverflow++;
-        if (h.overflow || h.pool_top * 4 > (unsigned long long)lcap * 3) {
-            rc = compact(c, cfg, s, &ncompact, lcap);
-            if (rc) return rc;
+        if (timed)
+            for (int i = 0; i < p2_launches; i++) {
+                float t = 0.f;
+                OPH_CHECK(hipEventElapsedTime(&t, c.p2_events[2 * i], c.p2_events[2 * i + 1]));
+                p2_ms += t;
+            }
+    } else {
+        // chunks of iterations; the first as long as the last reduction of
+        // this size took, so that a repeated reduction reads the control
+        // block once
+        const int next_k = cfg.chunk > 0 ? cfg.chunk : 3;
+        int k = (c.last_m == m && c.last_nnz == nnz && c.last_iters > 0) ? c.last_iters : 4;
+        int iter0 = 0;
+        while (true) {
+            k = std::min(k, 64);
+            GraphEntry *ge = nullptr;
+            if (sched == kSchedGraph && graph_for(c, sh, k, timed, &ge) != OPENPH_OK) {
+                // capture or instantiation refused: queue the same launches
+                (void)hipGetLastError();
+                sched = kSchedQueued;
+            }
+            if (ge) {
+                OPH_CHECK(hipGraphLaunch(ge->exec, s));
+                if (timed)
+                    OPH_CHECK(hipMemcpyAsync(c.h_stamps, c.d_stamps,
+                                             kStampRing * sizeof(unsigned long long),
+                                             hipMemcpyDeviceToHost, s));
+            } else {
+                if (timed && (rc = grow_events(c.chunk_events, 2 * (size_t)k))) return rc;
+                for (int q = 0; q < k; q++)
+                    if ((rc = enqueue_iteration(d, sh, s, timed ? c.chunk_events[2 * q] : nullptr,
+                                                timed ? c.chunk_events[2 * q + 1] : nullptr,
+                                                nullptr)))
+                        return rc;
+            }
+            if ((rc = read_ctrl(c, s))) return rc;
+            reads++;
+            // iterations of this chunk that reached Phase II: all that began,
+            // but the one that found convergence (or the cap) at its start
+            const int begun = h.iter - iter0;
+            const int ran2 =
+                begun - ((h.halt == kHaltConverged || h.halt == kHaltIters) ? 1 : 0);
+            if (timed && ge) {
+                for (int q = 0; q < ran2 && q < k; q++) {
+                    const int i = iter0 + q + 1;  // the iteration's stamps
+                    const unsigned long long a = c.h_stamps[(2 * (i - 1)) % kStampRing];
+                    const unsigned long long b = c.h_stamps[(2 * (i - 1) + 1) % kStampRing];
+                    if (c.wall_khz > 0 && b >= a) p2_ms += (float)((double)(b - a) / c.wall_khz);
+                }
+            } else if (timed) {
+                for (int q = 0; q < ran2 && q < k; q++) {
+                    float t = 0.f;
+                    OPH_CHECK(hipEventElapsedTime(&t, c.chunk_events[2 * q],
+                                                  c.chunk_events[2 * q + 1]));
+                    p2_ms += t;
+                }
+            }
+            p2_launches += std::max(ran2, 0);
+            iter0 = h.iter;
+            if (h.halt == kHaltConverged) break;
+            if (h.halt == kHaltCompact) {
+                if ((rc = compact(c, cfg, s, &ncompact, lcap))) return rc;
+                launch_set_halt(d, kHaltNone, s);
+            } else if (h.halt) {
+                return halt_status(h);
+            }
+            k = next_k;
         }
-        iter++;
     }
-    (void)converged;
+    const int num_iters = h.iter;
+    c.last_m = m;
+    c.last_nnz = nnz;
+    c.last_iters = num_iters;
+    if (tracking) {
+        // the state the reference tracks after its last iteration
+        // (pms.cu:94-100): ess_hat_final, update_classes_final
+        launch_final_ess(d, s);
+        launch_classes_final(d, s);
+        if ((rc = track_state(c, tr, num_iters - 1, s, m, ess_true))) return rc;
+        // scaling by the norms of low_true, over all m+1 entries
+        // (openph.cu:188-193)
+        for (int i = 0; i <= m; i++) {
+            if (tr->err_lone) tr->err_lone[i] = tr->err_lone[i] / norm1;
+            if (tr->err_linf) tr->err_linf[i] = tr->err_linf[i] / norminf;
+        }
+    }
     launch_final(d, low1, ess, s);
     OPH_CHECK(hipEventRecord(c.ev1, s));
-    OPH_CHECK(hipMemcpyAsync(&h, d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
-    OPH_CHECK(hipStreamSynchronize(s));
+    if ((rc = read_ctrl(c, s))) return rc;
     if (stats) {
-        stats->num_iters = iter;
+        stats->num_iters = num_iters;
         stats->compactions = ncompact;
-        stats->overflows = noverflow;
+        stats->overflows = h.overflows;
         stats->additions = (int64_t)h.additions;
         stats->merge_elems = (int64_t)h.merge_elems;
-        stats->pool_peak = (int64_t)std::max(peak, h.pool_top);
+        stats->pool_peak = (int64_t)std::max<unsigned long long>(h.pool_peak, (unsigned long long)nnz);
         float ms = 0.f;
         OPH_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
         stats->device_ms = ms;
         stats->m = m;
-        stats->lanes = lanes;
-        stats->group_lds = wlds;
+        stats->lanes = sh.lanes;
+        stats->group_lds = sh.wlds;
         stats->spills = h.spills;
         stats->handoffs = h.handoffs;
-        stats->phase2_launches = iter - 1;
-        float p2 = 0.f;
-        for (int i = 0; i < iter - 1; i++) {
-            float t = 0.f;
-            OPH_CHECK(hipEventElapsedTime(&t, c.p2_events[2 * i], c.p2_events[2 * i + 1]));
-            p2 += t;
-        }
-        stats->phase2_ms = p2;
+        stats->phase2_launches = p2_launches;
+        stats->phase2_ms = p2_ms;
+        stats->schedule = sched;
+        stats->host_reads = reads;
     }
-    if (time_track) {
-        time_track[0] = 0.f;
-        for (int i = 1; i < iter && (size_t)i < evs.size(); i++) {
+    if (iter_times) {
+        tr->time_track[0] = 0.f;
+        for (int i = 1; i < num_iters && i <= m; i++) {
             float ms = 0.f;
-            if (hipEventElapsedTime(&ms, evs[i - 1], evs[i]) == hipSuccess) time_track[i] = ms;
+            OPH_CHECK(hipEventElapsedTime(&ms, evs[2 * i], evs[2 * i + 1]));
+            tr->time_track[i] = ms;
         }
     }
     return OPENPH_OK;
@@ -352,7 +597,7 @@ int set_device(int device) {
 // Host-pointer run: stage through the context's device buffers.
 static int run_host(const char *algstr, const int *rows1, const int *cols1, int m, int nnz,
                     const openph_options *opt, int *low1, int *ess, openph_stats *stats,
-                    float *time_track) {
+                    const Track *tr) {
     if (!known_algo(algstr)) return OPENPH_EALGO;
     if (m < 0 || nnz < 0 || (nnz > 0 && (!rows1 || !cols1)) || (m > 0 && (!low1 || !ess)))
         return OPENPH_EINVAL;
@@ -372,7 +617,7 @@ static int run_host(const char *algstr, const int *rows1, const int *cols1, int 
         OPH_CHECK(hipMemcpyAsync(c->h_cols, cols1, sizeof(int) * nnz, hipMemcpyHostToDevice, s));
     }
     RunCfg cfg = cfg_from(opt, nnz);
-    rc = run_pms(*c, c->h_rows, c->h_cols, m, nnz, c->h_low, c->h_ess, s, cfg, stats, time_track);
+    rc = run_pms(*c, c->h_rows, c->h_cols, m, nnz, c->h_low, c->h_ess, s, cfg, stats, tr);
     if (rc) {
         (void)hipStreamSynchronize(s);
         return rc;
@@ -496,9 +741,19 @@ const char *openph_strerror(int status) {
 int openph_v2(const char *algstr, const int *rows1, const int *cols1, int m, int nnz,
               const int *low_true1, const openph_options *opt, int *low1, int *ess,
               float *trackers, openph_stats *stats) {
-    (void)low_true1;
-    float *tt = (trackers && opt && opt->track) ? trackers + 4 * ((size_t)m + 1) : nullptr;
-    return run_host(algstr, rows1, cols1, m, nnz, opt, low1, ess, stats, tt);
+    Track tr;
+    if (trackers && opt && opt->track && m >= 0) {
+        const size_t n = (size_t)m + 1;
+        tr.time_track = trackers + 4 * n;
+        if (low_true1) {
+            tr.low_true1 = low_true1;
+            tr.err_linf = trackers;
+            tr.err_lone = trackers + n;
+            tr.err_redu = trackers + 2 * n;
+            tr.err_ess = trackers + 3 * n;
+        }
+    }
+    return run_host(algstr, rows1, cols1, m, nnz, opt, low1, ess, stats, &tr);
 }
 
 int openph_pms_device(const int *rows1, const int *cols1, int m, int nnz, int *low1, int *ess,
@@ -517,17 +772,19 @@ void openph(char *algstr, int *h_rows_in, int *h_cols_in, int m, int col_width, 
             int nnz, int *h_low, int *h_ess, float *err_linf, float *err_lone, float *err_redu,
             float *err_ess, float *time_track, int *num_iters, int gpu_number) {
     (void)col_width;
-    (void)h_low_true;
-    (void)err_linf;
-    (void)err_lone;
-    (void)err_redu;
-    (void)err_ess;
     openph_options opt;
     std::memset(&opt, 0, sizeof(opt));
     opt.device = gpu_number;
     openph_stats st;
     std::memset(&st, 0, sizeof(st));
-    int rc = run_host(algstr, h_rows_in, h_cols_in, m, nnz, &opt, h_low, h_ess, &st, time_track);
+    Track tr;
+    tr.low_true1 = h_low_true;
+    tr.err_linf = err_linf;
+    tr.err_lone = err_lone;
+    tr.err_redu = err_redu;
+    tr.err_ess = err_ess;
+    tr.time_track = time_track;
+    int rc = run_host(algstr, h_rows_in, h_cols_in, m, nnz, &opt, h_low, h_ess, &st, &tr);
     if (num_iters) num_iters[0] = rc ? rc : st.num_iters;
 }
 

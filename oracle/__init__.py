@@ -68,7 +68,8 @@ def _lib():
             f.argtypes = [ctypes.c_int, i64p, i32p, ctypes.c_int, i32p, i32p, i64p]
         lib.orc_pms.restype = ctypes.c_int
         lib.orc_pms.argtypes = [ctypes.c_int, i64p, i32p, i32p, i32p, i32p, i64p,
-                                ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+                                ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_void_p, ctypes.c_void_p]
         lib.orc_ph_row.restype = ctypes.c_int
         lib.orc_ph_row.argtypes = [ctypes.c_int, i64p, i32p, i32p, i32p, i64p]
         _LIB = lib
@@ -130,7 +131,11 @@ def _safe_cw(b, col_width):
     return int(min(col_width, cap))
 
 
-def pms(b, max_iters: int = 0, iter_stats: bool = False, snapshot: bool = False) -> OracleResult:
+def pms(b, max_iters: int = 0, iter_stats: bool = False, snapshot: bool = False,
+        low_true=None) -> OracleResult:
+    """pms; with low_true (0-based, -1 = none) the result also carries `trk`,
+    an (num_iters, 4) int64 array of the tracked integers per iteration:
+    sum |low - low_true|, max |low - low_true|, #unreduced, #ess."""
     colptr, rows = _csc(b)
     m = b.m
     low = np.empty(m, np.int32)
@@ -140,11 +145,22 @@ def pms(b, max_iters: int = 0, iter_stats: bool = False, snapshot: bool = False)
     cap = 100000
     its = np.zeros(5 * cap, np.int64) if iter_stats else None
     ptr = its.ctypes.data if its is not None else None
-    _check(_lib().orc_pms(m, colptr, rows, low, ess, nit, st, ptr, cap, max_iters, int(snapshot)), "pms")
+    lt = None
+    trk = None
+    if low_true is not None:
+        lt = np.ascontiguousarray(low_true, dtype=np.int32)
+        if lt.shape[0] != m:
+            raise ValueError("low_true must have m entries")
+        trk = np.zeros(4 * (m + 1), np.int64)
+    _check(_lib().orc_pms(m, colptr, rows, low, ess, nit, st, ptr, cap, max_iters,
+                          int(snapshot), lt.ctypes.data if lt is not None else None,
+                          trk.ctypes.data if trk is not None else None), "pms")
     per = None
     if its is not None:
         per = its.reshape(-1, 5)[: int(st[3])].copy()
-    return OracleResult(low, ess, st, int(nit[0]), per)
+    res = OracleResult(low, ess, st, int(nit[0]), per)
+    res.trk = trk.reshape(-1, 4)[: int(nit[0])].copy() if trk is not None else None
+    return res
 
 
 def ph_row(b) -> OracleResult:

@@ -305,9 +305,35 @@ static void left_beta(int m, const int64_t *colptr, const int32_t *rows,
 /* iter_stats (optional): per iteration 5 int64 -
  *   additions, merge elements, columns marked in Phase I, columns that
  *   entered Phase II's loop, longest chain of additions of one column. */
+/* The quantities track() reads at the end of an iteration
+ * (tracker.cu:92-134), as exact integers: q[0] = sum_j |low_j - lt_j|,
+ * q[1] = max_j |low_j - lt_j| (both 0-based, openph.cu:113), q[2] =
+ * #{classes_j == 0}, q[3] = #{ess_j == 1}. */
+static void orc_track(int m, const int32_t *low, const int32_t *lt, const int32_t *classes,
+                      const int32_t *ess, int64_t *q) {
+    int64_t l1 = 0, linf = 0, un = 0, es = 0;
+    for (int j = 0; j < m; j++) {
+        int64_t d = (int64_t)low[j] - (int64_t)lt[j];
+        if (d < 0) d = -d;
+        l1 += d;
+        if (d > linf) linf = d;
+        un += classes[j] == 0;
+        es += ess[j] == 1;
+    }
+    q[0] = l1;
+    q[1] = linf;
+    q[2] = un;
+    q[3] = es;
+}
+
+/* low_true0 (0-based, may be NULL) and trk (4 x (m+1) entries, may be NULL):
+ * when both are given, trk[4*i .. 4*i+3] receives orc_track of the state
+ * tracked at iteration i (pms.cu:16-19 before Phase 0, pms.cu:106-109 after
+ * every iteration). */
 int orc_pms(int m, const int64_t *colptr, const int32_t *rows, int32_t *low,
             int32_t *ess, int32_t *num_iters, int64_t *stats, int64_t *iter_stats,
-            int max_iter_stats, int max_iters, int snapshot) {
+            int max_iter_stats, int max_iters, int snapshot, const int32_t *low_true0,
+            int64_t *trk) {
     if (m < 0) return ORC_EINPUT;
     int rc = ORC_OK;
     dyn_store s;
@@ -331,6 +357,8 @@ int orc_pms(int m, const int64_t *colptr, const int32_t *rows, int32_t *low,
     left_beta(m, colptr, rows, left, beta);
     for (int j = 0; j < m; j++) { arglow[j] = -1; classes[j] = 0; ess[j] = 1; }
     int64_t st[ST_N] = {0, 0, 0, 0};
+    const int tracking = low_true0 && trk;
+    if (tracking) orc_track(m, low, low_true0, classes, ess, trk);
 
     /* Phase 0: mark_pivots_and_clear (pms_functions.cu:33-53) */
     for (int j = 0; j < m; j++) {
@@ -434,6 +462,7 @@ int orc_pms(int m, const int64_t *colptr, const int32_t *rows, int32_t *low,
                 else if (ess[t] == 1) classes[t] = 2;
             }
         }
+        if (tracking && iter <= m) orc_track(m, low, low_true0, classes, ess, trk + 4 * (int64_t)iter);
         if (iter_stats && iter - 1 < max_iter_stats) {
             int64_t *q = iter_stats + 5 * (int64_t)(iter - 1);
             q[0] = st[ST_ADD] - add0;

@@ -226,3 +226,36 @@ def test_oracle_slot_overflow_is_an_error(oracle_lib):
     st = np.zeros(4, np.int64)
     rc = lib.orc_twist(b.m, colptr, rows, 1, low, ess, st)
     assert rc == oracle.ORC_EOVERFLOW
+
+
+@pytest.mark.parametrize("name", ["vr12", "random0", "random2", "vr20_sphere", "vr40"])
+def test_oracle_trackers(name, oracle_lib):
+    """The integers behind the reference's trackers (tracker.cu:92-134) per
+    iteration, pinned at both ends by closed forms: before Phase 0 every
+    column is unreduced and essential and the low distance is that of the
+    input's last rows; after the last iteration low equals low_true, nothing
+    is unreduced (update_classes_final classifies every column) and the
+    essential estimate is exact. In between, classes and ess only ever move
+    away from 0 and 1, so both counts never increase."""
+    f = np.load(os.path.join(GOLD, name + ".npz"))
+    b = _gen(name)
+    lt = f["low"].astype(np.int32)
+    res = oracle.pms(b, snapshot=True, low_true=lt)
+    trk = res.trk
+    assert trk.shape == (res.num_iters, 4)
+    m = b.m
+    counts = np.diff(b.colptr)
+    low0 = np.where(counts > 0, b.rows[np.maximum(b.colptr[1:] - 1, 0)], -1)
+    d0 = np.abs(low0.astype(np.int64) - lt)
+    assert trk[0].tolist() == [d0.sum(), d0.max(), m, m]
+    paired = np.zeros(m, bool)
+    j = np.flatnonzero(lt >= 0)
+    paired[j] = True
+    paired[lt[j]] = True
+    assert trk[-1].tolist() == [0, 0, 0, int((~paired).sum())]
+    assert (np.diff(trk[:, 2]) <= 0).all() and (np.diff(trk[:, 3]) <= 0).all()
+    # a wrong low_true shows up in the distances
+    lt2 = lt.copy()
+    lt2[j[0]] = -1
+    bad = oracle.pms(b, snapshot=True, low_true=lt2).trk
+    assert bad[-1][0] == lt[j[0]] + 1 and bad[-1][1] == lt[j[0]] + 1
