@@ -75,10 +75,11 @@ def cpu_baseline(b, budget_s: float = 10.0):
     }, low
 
 
-def batched_leg(count: int = 512, reps: int = 3):
+def batched_leg(count: int = 512, reps: int = 3, tune: bool = True):
     """512 VR 2-skeleta (N ~ U{20..80}, mixed shapes) reduced (a) by one
-    batched call and (b) by `count` sequential plain calls, inputs resident in
-    HBM for both; outputs compared bit for bit."""
+    batched call with the setting the autotuner picks on this batch and
+    (b) by `count` sequential plain calls with the default setting, inputs
+    resident in HBM for both; outputs compared bit for bit."""
     import torch
 
     import openph_amd
@@ -109,14 +110,22 @@ def batched_leg(count: int = 512, reps: int = 3):
             openph_amd.pms_device(rows[a:b_], cols[a:b_], y - x, low_p[x:y], ess_p[x:y],
                                   stream=stream)
 
+    best, tune_rows, tune_s = None, [], 0.0
+    if tune:
+        t0 = time.perf_counter()
+        best, tune_rows = openph_amd.autotune_batch_device(ms, nnzs, rows, cols, low_b, ess_b,
+                                                           stream=stream, reps=2)
+        tune_s = time.perf_counter() - t0
     # warm both paths once
-    st_b = openph_amd.pms_batch_device(ms, nnzs, rows, cols, low_b, ess_b, stream=stream)
+    st_b = openph_amd.pms_batch_device(ms, nnzs, rows, cols, low_b, ess_b, stream=stream,
+                                       options=best)
     plain_pass()
     torch.cuda.synchronize()
     tb = []
     for _ in range(reps):
         t0 = time.perf_counter()
-        st_b = openph_amd.pms_batch_device(ms, nnzs, rows, cols, low_b, ess_b, stream=stream)
+        st_b = openph_amd.pms_batch_device(ms, nnzs, rows, cols, low_b, ess_b, stream=stream,
+                                           options=best)
         torch.cuda.synchronize()
         tb.append(time.perf_counter() - t0)
     t0 = time.perf_counter()
@@ -131,69 +140,83 @@ def batched_leg(count: int = 512, reps: int = 3):
         "batched_cols_per_s": M / b_s, "batched_ms": b_s * 1e3,
         "plain_sequential_cols_per_s": M / tp, "plain_sequential_ms": tp * 1e3,
         "speedup_batched_vs_plain": tp / b_s,
+        "tuned": ({"lanes_per_column": st_b.lanes, "group_lds": st_b.group_lds,
+                   "slot_elems": best.slot_elems or 32, "schedule": st_b.schedule,
+                   "tuning_s": tune_s, "candidates": len(tune_rows),
+                   "all_candidates_equal": all(r["same"] for r in tune_rows)}
+                  if tune else None),
         "batched_num_iters": st_b.num_iters, "batched_additions": st_b.additions,
         "batched_merge_elems": st_b.merge_elems,
         "batched_equals_plain": same, "host_generation_s": gen_s,
     }
 
 
-def served_leg(n_pool: int = 24, n_requests: int = 72, workers: int = 4,
-               loads=(1.0, 8.0), seed: int = 0):
-    """Served pms on the configs[4] mix: 70% VR 2-skeleta of N ~ U{30..200}
-    uniform points, 30% cubical lower-star grids of side ~ U{16..96}.
+def served_leg(workers: int = 4, seed: int = 0):
+    """Served pms (openph_server_*: worker streams, admission by pool
+    footprint, continuous batching) against the plain drop-in (openph_v2,
+    host COO in, host low/ess out, one request after another), on two
+    request streams:
 
-    A pool of `n_pool` requests is generated once (seed 0); the stream cycles
-    through it. Plain: the requests one after another through the drop-in
-    (openph_v2, host COO in, host low/ess out). Served: Poisson arrivals at a
-    fixed QPS = load x the request rate plain sustains, submitted to a Server
-    with `workers` device workers; latency = submit -> results on the host.
-    At load 1 the QPS is one the plain path could just keep up with (the
-    latency figure); at load 8 arrivals outpace the server, so the span
-    measures its throughput. An open arrival stream cannot finish faster than
-    it arrives, so throughput is only read at the saturating load."""
-    import openph_amd
+    * mix512: the 512 VR 2-skeleta of the batched leg as 512 requests;
+    * vr_cubical: 72 requests cycling over 24 drawn once (seed 0): 70% VR
+      2-skeleta of N ~ U{30..200} uniform points, 30% cubical lower-star
+      grids of side ~ U{16..96} (Gaussian field smoothed with sigma 2).
+
+    For each: throughput with every request submitted at once (the server
+    saturated; sum of m / (last result - first submit)), and p50/p99 latency
+    (submit -> results on the host) under Poisson arrivals at the rate the
+    plain drop-in sustains. Every result is compared with the same matrix
+    reduced alone."""
     from openph_amd import complexes as C
 
-    rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
-    pool = [C.served_request(rng) for _ in range(n_pool)]
-    coo = [b.coo1() for b in pool]
+    mix = C.batched_mix(512, seed=0)
+    rng = np.random.default_rng(seed)
+    pool = [C.served_request(rng) for _ in range(24)]
     gen_s = time.perf_counter() - t0
-    ref = [openph_amd.reduce(b, "pms")[:2] for b in pool]  # warm-up and reference outputs
-    seq = [i % n_pool for i in range(n_requests)]
+    out = {"workers": workers, "host_generation_s": gen_s}
+    out["mix512"] = _served_stream(mix, list(range(len(mix))), workers, seed,
+                                   "the 512 VR 2-skeleta of the batched leg")
+    out["vr_cubical"] = _served_stream(
+        pool, [i % len(pool) for i in range(72)], workers, seed,
+        "72 requests cycling over 24 (seed 0): 70% VR N~U{30..200}, 30% cubical side~U{16..96}")
+    return out
+
+
+def _served_stream(pool, seq, workers, seed, what):
+    import openph_amd
+
+    coo = [b.coo1() for b in pool]
+    ref = [openph_amd.reduce(b, "pms")[:2] for b in pool]  # warm-up and outputs alone
     cols = sum(pool[i].m for i in seq)
     t0 = time.perf_counter()
     for i in seq:
         openph_amd.reduce(pool[i], "pms")
     t_plain = time.perf_counter() - t0
-    runs = {}
-    for load in loads:
-        runs[f"load_{load:g}x"] = _served_run(pool, coo, ref, seq, cols, t_plain, load, workers,
-                                              np.random.default_rng(seed + 1))
-    sat = runs[f"load_{max(loads):g}x"]
-    lat = runs[f"load_{min(loads):g}x"]
+    burst = _served_run(pool, coo, ref, seq, cols, None, workers, np.random.default_rng(seed + 1))
+    qps = len(seq) / t_plain
+    paced = _served_run(pool, coo, ref, seq, cols, qps, workers, np.random.default_rng(seed + 1))
     return {
-        "workload": (f"{n_requests} requests cycling over {n_pool} (seed {seed}): 70% VR "
-                     "N~U{30..200}, 30% cubical side~U{16..96}; Poisson arrivals"),
-        "workers": workers, "columns": cols,
+        "workload": what, "requests": len(seq), "columns": cols,
         "plain_sequential_cols_per_s": cols / t_plain, "plain_sequential_s": t_plain,
-        "served_cols_per_s": sat["served_cols_per_s"],
-        "speedup_served_vs_plain": sat["served_cols_per_s"] / (cols / t_plain),
-        "latency_ms_p50": lat["latency_ms_p50"], "latency_ms_p99": lat["latency_ms_p99"],
-        "qps_for_latency": lat["qps"], "runs": runs,
-        "served_equals_alone": all(r["served_equals_alone"] for r in runs.values()),
-        "host_generation_s": gen_s,
+        "served_cols_per_s": burst["served_cols_per_s"],
+        "speedup_served_vs_plain": burst["served_cols_per_s"] / (cols / t_plain),
+        "latency_ms_p50": paced["latency_ms_p50"], "latency_ms_p99": paced["latency_ms_p99"],
+        "latency_qps": qps, "burst": burst, "poisson": paced,
+        "served_equals_alone": burst["served_equals_alone"] and paced["served_equals_alone"],
     }
 
 
-def _served_run(pool, coo, ref, seq, cols, t_plain, load, workers, rng):
+def _served_run(pool, coo, ref, seq, cols, qps, workers, rng):
+    """Submit `seq` to a fresh server, all at once (qps None) or with Poisson
+    arrivals at `qps`; wait for every result and compare it with `ref`."""
     import threading
 
     from openph_amd.server import Server
 
     n_requests = len(seq)
-    qps = load * n_requests / t_plain
-    arrivals = np.cumsum(rng.exponential(1.0 / qps, n_requests))
+    arrivals = (np.zeros(n_requests) if qps is None
+                else np.cumsum(rng.exponential(1.0 / qps, n_requests)))
     tickets = [None] * n_requests
     with Server(workers=workers) as srv:
         def submitter():
@@ -210,7 +233,7 @@ def _served_run(pool, coo, ref, seq, cols, t_plain, load, workers, rng):
         lat, first, last, same = [], None, None, True
         for k, i in enumerate(seq):
             while tickets[k] is None:
-                time.sleep(0.0005)
+                time.sleep(0.0002)
             low, ess, info = srv.wait(tickets[k])
             same &= bool(np.array_equal(low, ref[i][0]) and np.array_equal(ess, ref[i][1]))
             lat.append(info.done_s - info.submit_s)
@@ -221,8 +244,7 @@ def _served_run(pool, coo, ref, seq, cols, t_plain, load, workers, rng):
     span = last - first
     lat_ms = np.array(lat) * 1e3
     return {
-        "qps": qps, "offered_load_vs_plain": load,
-        "served_cols_per_s": cols / span, "served_span_s": span,
+        "qps": qps, "served_cols_per_s": cols / span, "span_s": span,
         "latency_ms_p50": float(np.percentile(lat_ms, 50)),
         "latency_ms_p99": float(np.percentile(lat_ms, 99)),
         "latency_ms_max": float(lat_ms.max()),
@@ -262,14 +284,18 @@ def main():
     low = torch.empty(b.m, dtype=torch.int32, device=dev)
     ess = torch.empty(b.m, dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream(dev)
+    # queued chunks of iterations: Phase II is bracketed by HIP events on the
+    # stream it runs on (graph replay would time it with device-clock stamps)
+    opt = openph_amd.Options()
+    opt.schedule = openph_amd.Options.QUEUED
     print(f"bench: torus100 m={b.m} nnz={b.nnz}, warmup {args.warmup}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
-        openph_amd.pms_device(dr, dc, b.m, low, ess, stream=stream)
+        openph_amd.pms_device(dr, dc, b.m, low, ess, stream=stream, options=opt)
     torch.cuda.synchronize()
     stats = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        stats.append(openph_amd.pms_device(dr, dc, b.m, low, ess, stream=stream))
+        stats.append(openph_amd.pms_device(dr, dc, b.m, low, ess, stream=stream, options=opt))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     print(f"bench: {args.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
@@ -331,7 +357,8 @@ def main():
                      "avg_launch_ms": p2_ms / max(1, launches)},
         "hbm_alg": {"bytes": b_alg, "device_ms": dev_ms,
                     "GBps": b_alg / (dev_ms * 1e-3) / 1e9 if dev_ms > 0 else 0.0},
-        "pms": {"num_iters": st.num_iters, "additions": st.additions,
+        "pms": {"num_iters": st.num_iters, "additions": st.additions, "schedule": st.schedule,
+                "host_reads": st.host_reads,
                 "merge_elems": st.merge_elems, "compactions": st.compactions,
                 "pool_peak": st.pool_peak, "additions_per_s": st.additions * args.steps / wall,
                 "parity_vs_golden": ok},

@@ -170,6 +170,38 @@ int openph_pms_batch_device(int count, const int *ms, const int *nnzs, const int
                             const int *cols1, int *low1, int *ess, void *stream,
                             const openph_options *opt, openph_stats *stats);
 
+/* One candidate measured by the autotuner. */
+typedef struct openph_tune_row {
+    int lanes_per_column;    /* Phase II lanes per column                    */
+    int group_lds;           /* LDS entries per lane-group buffer            */
+    int slot_elems;          /* minimum pool segment per column              */
+    int schedule;            /* loop schedule (1, 2 or 3)                    */
+    float wall_ms;           /* median wall time of one batched reduction    */
+    int same;                /* 1: low/ess equal the reference candidate's   */
+} openph_tune_row;
+
+/*
+ * Autotuner for the batched path (no counterpart in the reference, whose
+ * threads per column and slot width p are fixed by the caller). Reduces the
+ * batch held on the device (arguments as openph_pms_batch_device) with
+ * candidate settings, one untimed run and `reps` timed runs each (median of
+ * the host wall time, inputs resident), in three stages:
+ *   1. Phase II shape: every compiled {lanes_per_column, group_lds} and one
+ *      lane per column, with base's pool granularity and schedule;
+ *   2. pool granularity: slot_elems in {8, 16, 32, 64} with the best shape;
+ *   3. loop schedule: 1, 2 and 3 with the best shape and granularity.
+ * Every candidate's low/ess must equal the first candidate's (base
+ * options); a difference returns OPENPH_EDEVICE. *best receives base with
+ * the fastest setting; pass it to later calls (or to a server's
+ * openph_server_options.reduce). report (may be NULL) receives up to
+ * max_report rows in the order measured, *nreport their number.
+ * low1/ess end with the outputs of the last run.
+ */
+int openph_autotune_batch_device(int count, const int *ms, const int *nnzs, const int *rows1,
+                                 const int *cols1, int *low1, int *ess, void *stream,
+                                 const openph_options *base, int reps, openph_options *best,
+                                 openph_tune_row *report, int max_report, int *nreport);
+
 /*
  * Served reduction (no counterpart in the reference, which is a blocking
  * MATLAB call). A server owns `workers` device workspaces, each with its own

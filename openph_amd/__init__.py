@@ -25,7 +25,8 @@ import numpy as np
 
 __all__ = [
     "OpenPHError", "Options", "Stats", "lib", "build", "ph", "openph", "reduce",
-    "pms_device", "pms_batch", "pms_batch_device", "ALGORITHMS",
+    "pms_device", "pms_batch", "pms_batch_device", "autotune_batch_device", "TuneRow",
+    "ALGORITHMS",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -87,6 +88,21 @@ class Stats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class TuneRow(ctypes.Structure):
+    """openph_tune_row (include/openph.h)."""
+    _fields_ = [
+        ("lanes_per_column", ctypes.c_int),
+        ("group_lds", ctypes.c_int),
+        ("slot_elems", ctypes.c_int),
+        ("schedule", ctypes.c_int),
+        ("wall_ms", ctypes.c_float),
+        ("same", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 def build(force: bool = True) -> str:
     """Compile libopenph.so for gfx950 with hipcc (in-tree)."""
     args = ["make", "-C", os.path.join(_HERE, "csrc"), "-j8"]
@@ -128,6 +144,12 @@ def lib():
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_void_p, ctypes.POINTER(Options),
                                               ctypes.POINTER(Stats)]
+        L.openph_autotune_batch_device.restype = ctypes.c_int
+        L.openph_autotune_batch_device.argtypes = [
+            ctypes.c_int, c_int_p, c_int_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Options), ctypes.c_int,
+            ctypes.POINTER(Options), ctypes.POINTER(TuneRow), ctypes.c_int,
+            ctypes.POINTER(ctypes.c_int)]
         L.openph_strerror.restype = ctypes.c_char_p
         L.openph_strerror.argtypes = [ctypes.c_int]
         L.openph_version.restype = ctypes.c_char_p
@@ -276,3 +298,23 @@ def pms_batch_device(ms, nnzs, rows1, cols1, low1, ess, stream=None,
     if rc != 0:
         raise OpenPHError(rc, "pms_batch_device")
     return st
+
+
+def autotune_batch_device(ms, nnzs, rows1, cols1, low1, ess, stream=None,
+                          base: Options | None = None, reps: int = 3):
+    """Autotune the batched path on device tensors (arguments as
+    ``pms_batch_device``). Returns (best Options, list of measured rows as
+    dicts); every candidate's output is checked against the first's."""
+    ms = _i32(ms)
+    nnzs = _i32(nnzs)
+    best = Options()
+    rows = (TuneRow * 64)()
+    n = ctypes.c_int(0)
+    s = stream.cuda_stream if stream is not None and hasattr(stream, "cuda_stream") else stream
+    rc = lib().openph_autotune_batch_device(
+        len(ms), _iptr(ms), _iptr(nnzs), rows1.data_ptr(), cols1.data_ptr(), low1.data_ptr(),
+        ess.data_ptr(), s, ctypes.byref(base or Options()), int(reps), ctypes.byref(best), rows,
+        64, ctypes.byref(n))
+    if rc != 0:
+        raise OpenPHError(rc, "autotune_batch_device")
+    return best, [rows[i].as_dict() for i in range(n.value)]

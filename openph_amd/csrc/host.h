@@ -170,6 +170,8 @@ struct RunCfg {
 };
 
 RunCfg cfg_from(const openph_options *opt, int nnz);
+// The workspace of a device, created on first use and kept for the process.
+Context *context_for(int device);
 // Sizes a workspace for batches of up to m columns, nnz entries and a pool of
 // pool_elems entries, so that serving does not allocate (hipFree waits for
 // the whole device).

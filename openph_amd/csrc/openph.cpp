@@ -26,7 +26,7 @@
 
 namespace oph {
 
-static Context *context_for(int device) {
+Context *context_for(int device) {
     static std::mutex gmu;
     static std::vector<Context *> ctxs;
     std::lock_guard<std::mutex> g(gmu);
@@ -391,10 +391,16 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
         sh.long_grid = c.num_cu;
     }
     // the schedule: per-iteration reads when the trackers or the iteration
-    // times are wanted, else queued chunks
+    // times are wanted; else graph replay when the last reduction on this
+    // workspace had the same size (its graphs are likely cached: a capture
+    // costs more than it saves once), queued chunks otherwise
     int sched = cfg.schedule;
     const bool per_iter = tracking || (tr && tr->time_track);
-    if (per_iter || sched == kSchedAuto) sched = per_iter ? kSchedSync : kSchedQueued;
+    if (per_iter) {
+        sched = kSchedSync;
+    } else if (sched == kSchedAuto) {
+        sched = (c.last_m == m && c.last_nnz == nnz) ? kSchedGraph : kSchedQueued;
+    }
     const bool timed = stats != nullptr;
 
     OPH_CHECK(hipEventRecord(c.ev0, s));

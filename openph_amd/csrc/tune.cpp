@@ -1,0 +1,167 @@
+// Autotuner of the batched pms path: Phase II lanes per column and LDS
+// buffer, pool granularity and loop schedule, chosen by measurement on the
+// caller's batch with the results checked against each other.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "host.h"
+
+namespace oph {
+
+namespace {
+
+struct Tuner {
+    Context &c;
+    int count;
+    const int *ms, *nnzs, *rows1, *cols1;
+    int *low1, *ess;
+    hipStream_t s;
+    int reps;
+    size_t M = 0;
+    int nnz_total = 0;
+    std::vector<int> ref_low, ref_ess, cur_low, cur_ess;
+    openph_tune_row *report;
+    int max_report;
+    int nrows = 0;
+
+    int copy_out(std::vector<int> &l, std::vector<int> &e) {
+        l.resize(M);
+        e.resize(M);
+        if (!M) return OPENPH_OK;
+        OPH_CHECK(hipMemcpyAsync(l.data(), low1, sizeof(int) * M, hipMemcpyDeviceToHost, s));
+        OPH_CHECK(hipMemcpyAsync(e.data(), ess, sizeof(int) * M, hipMemcpyDeviceToHost, s));
+        OPH_CHECK(hipStreamSynchronize(s));
+        return OPENPH_OK;
+    }
+
+    int once(const openph_options &o, double *ms_out) {
+        RunCfg cfg = cfg_from(&o, nnz_total);
+        openph_stats st;
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc = run_batch(c, count, ms, nnzs, rows1, cols1, low1, ess, s, cfg, &st);
+        if (rc) return rc;
+        OPH_CHECK(hipStreamSynchronize(s));
+        *ms_out = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                      .count();
+        return OPENPH_OK;
+    }
+
+    // One candidate: an untimed run (graph capture, workspace growth), the
+    // comparison with the reference outputs, then `reps` timed runs (fewer
+    // when the first is already 4x slower than the best so far).
+    int measure(const openph_options &o, double best_ms, double *med) {
+        double t = 0;
+        int rc = once(o, &t);
+        if (rc) return rc;
+        bool same = true;
+        if (ref_low.empty() && M) {
+            if ((rc = copy_out(ref_low, ref_ess))) return rc;
+        } else if (M) {
+            if ((rc = copy_out(cur_low, cur_ess))) return rc;
+            same = cur_low == ref_low && cur_ess == ref_ess;
+        }
+        std::vector<double> ts;
+        for (int r = 0; r < reps; r++) {
+            if ((rc = once(o, &t))) return rc;
+            ts.push_back(t);
+            if (t > 4 * best_ms) break;
+        }
+        std::sort(ts.begin(), ts.end());
+        *med = ts[ts.size() / 2];
+        if (report && nrows < max_report) {
+            openph_tune_row &row = report[nrows];
+            row.lanes_per_column = o.lanes_per_column;
+            row.group_lds = o.group_lds;
+            row.slot_elems = o.slot_elems;
+            row.schedule = o.schedule;
+            row.wall_ms = (float)*med;
+            row.same = same ? 1 : 0;
+        }
+        nrows++;
+        return same ? OPENPH_OK : OPENPH_EDEVICE;
+    }
+};
+
+}  // namespace
+
+}  // namespace oph
+
+using namespace oph;
+
+extern "C" int openph_autotune_batch_device(int count, const int *ms, const int *nnzs,
+                                            const int *rows1, const int *cols1, int *low1,
+                                            int *ess, void *stream, const openph_options *base,
+                                            int reps, openph_options *best,
+                                            openph_tune_row *report, int max_report,
+                                            int *nreport) {
+    if (count < 0 || (count > 0 && (!ms || !nnzs)) || !best || reps < 1 || max_report < 0 ||
+        (max_report > 0 && !report))
+        return OPENPH_EINVAL;
+    openph_options b0;
+    std::memset(&b0, 0, sizeof(b0));
+    if (base) b0 = *base;
+    int rc = set_device(b0.device);
+    if (rc) return rc;
+    Context *c = context_for(b0.device);
+    std::lock_guard<std::mutex> g(c->mu);
+    long long M = 0, N = 0;
+    for (int k = 0; k < count; k++) {
+        if (ms[k] < 0 || nnzs[k] < 0) return OPENPH_EINVAL;
+        M += ms[k];
+        N += nnzs[k];
+    }
+    if (M > 0x7fffffffLL || N > 0x7fffffffLL) return OPENPH_EINVAL;
+    Tuner t{*c, count, ms, nnzs, rows1, cols1, low1, ess, (hipStream_t)stream, reps};
+    t.M = (size_t)M;
+    t.nnz_total = (int)N;
+    t.report = report;
+    t.max_report = max_report;
+
+    double best_ms = 1e30, ms_cur = 0;
+    openph_options cur = b0;
+    // the reference candidate: base as given
+    if ((rc = t.measure(cur, best_ms, &ms_cur))) return rc;
+    openph_options win = cur;
+    best_ms = ms_cur;
+    auto consider = [&](const openph_options &o) -> int {
+        double m2 = 0;
+        int r = t.measure(o, best_ms, &m2);
+        if (r) return r;
+        if (m2 < best_ms) {
+            best_ms = m2;
+            win = o;
+        }
+        return OPENPH_OK;
+    };
+    // 1. Phase II shape
+    static const int shapes[][2] = {{1, 0},    {4, 32},    {4, 64},   {8, 32},  {8, 64},
+                                    {8, 128},  {16, 64},   {16, 128}, {16, 256}, {32, 256},
+                                    {32, 512}, {64, 512},  {64, 1024}};
+    for (const auto &sh : shapes) {
+        openph_options o = b0;
+        o.lanes_per_column = sh[0];
+        o.group_lds = sh[1];
+        if ((rc = consider(o))) return rc;
+    }
+    // 2. pool granularity
+    const openph_options after_shape = win;
+    for (int slot : {8, 16, 32, 64}) {
+        openph_options o = after_shape;
+        o.slot_elems = slot;
+        if ((rc = consider(o))) return rc;
+    }
+    // 3. schedule
+    const openph_options after_slot = win;
+    for (int sched : {kSchedSync, kSchedQueued, kSchedGraph}) {
+        openph_options o = after_slot;
+        o.schedule = sched;
+        if ((rc = consider(o))) return rc;
+    }
+    *best = win;
+    if (nreport) *nreport = std::min(t.nrows, max_report);
+    return OPENPH_OK;
+}

@@ -63,3 +63,46 @@ def test_mismatched_coo_lengths_rejected():
 def test_library_has_gfx950_code_object():
     blob = open(openph_amd._SO, "rb").read()
     assert b"gfx950" in blob
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of the public structs have the C layout (sizes and
+    field offsets from a C program compiled against include/openph.h)."""
+    from openph_amd.server import RequestInfo, ServerOptions
+    structs = {
+        "openph_options": openph_amd.Options,
+        "openph_stats": openph_amd.Stats,
+        "openph_tune_row": openph_amd.TuneRow,
+        "openph_server_options": ServerOptions,
+        "openph_request_info": RequestInfo,
+    }
+    src = ["#include <stdio.h>", "#include <stddef.h>", '#include "openph.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        src.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            src.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    src.append("return 0; }")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    rc = os.system(f"gcc -I{os.path.join(ROOT, 'include')} {c} -o {exe}")
+    assert rc == 0
+    got = dict(line.rsplit(" ", 1) for line in os.popen(str(exe)).read().splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)
+
+
+def test_autotune_argument_checks_on_host():
+    L = openph_amd.lib()
+    best = openph_amd.Options()
+    ms = np.zeros(1, np.int32)
+    ip = ms.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    # reps < 1, a missing best, a negative report size: refused before any device work
+    assert L.openph_autotune_batch_device(1, ip, ip, None, None, None, None, None, None, 0,
+                                          ctypes.byref(best), None, 0, None) == -1
+    assert L.openph_autotune_batch_device(1, ip, ip, None, None, None, None, None, None, 1,
+                                          None, None, 0, None) == -1
+    assert L.openph_autotune_batch_device(1, ip, ip, None, None, None, None, None, None, 1,
+                                          ctypes.byref(best), None, -1, None) == -1

@@ -239,3 +239,31 @@ def test_low_true_out_of_range_is_rejected():
     with pytest.raises(openph_amd.OpenPHError) as e:
         openph_amd.ph("pms", r, c, b.m, 8, lt)
     assert e.value.status == -1
+
+
+def test_autotune_batch_checks_every_candidate_and_returns_a_working_setting():
+    say("autotune")
+    import torch
+    mats = C.batched_mix(32, seed=7)
+    ms = np.array([b.m for b in mats], np.int32)
+    nn = np.array([b.nnz for b in mats], np.int32)
+    rr, cc = zip(*(b.coo1() for b in mats))
+    dev = torch.device("cuda:0")
+    rows = torch.from_numpy(np.concatenate(rr)).to(dev)
+    cols = torch.from_numpy(np.concatenate(cc)).to(dev)
+    M = int(ms.sum())
+    low = torch.empty(M, dtype=torch.int32, device=dev)
+    ess = torch.empty_like(low)
+    best, rows_ = openph_amd.autotune_batch_device(ms, nn, rows, cols, low, ess, reps=1)
+    assert len(rows_) == 1 + 13 + 4 + 3
+    assert all(r["same"] == 1 for r in rows_)
+    assert min(r["wall_ms"] for r in rows_) > 0
+    st = openph_amd.pms_batch_device(ms, nn, rows, cols, low, ess, options=best)
+    torch.cuda.synchronize()
+    assert st.lanes == (best.lanes_per_column or st.lanes)
+    got = low.cpu().numpy()
+    o = 0
+    for b in mats:
+        ref = oracle.pms(b)
+        assert np.array_equal(got[o:o + b.m] - 1, ref.low)
+        o += b.m
