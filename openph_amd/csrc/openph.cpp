@@ -198,10 +198,15 @@ static int enqueue_iteration(const Dev &d, const Shape &sh, hipStream_t s, hipEv
 }
 
 // OPENPH_DEBUG=1 in the environment reports recoverable runtime refusals
-// (a graph capture that falls back to queued launches) on stderr.
-static void debug_note(const char *what, hipError_t e) {
+// (a graph capture that falls back to queued launches) and the Phase II
+// time of every iteration of the sync schedule on stderr.
+static bool debug_on() {
     static const bool on = std::getenv("OPENPH_DEBUG") != nullptr;
-    if (on) std::fprintf(stderr, "openph: %s: %s\n", what, hipGetErrorString(e));
+    return on;
+}
+
+static void debug_note(const char *what, hipError_t e) {
+    if (debug_on()) std::fprintf(stderr, "openph: %s: %s\n", what, hipGetErrorString(e));
 }
 
 static void drop_graph(GraphEntry &g) {
@@ -468,6 +473,8 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
                 float t = 0.f;
                 OPH_CHECK(hipEventElapsedTime(&t, c.p2_events[2 * i], c.p2_events[2 * i + 1]));
                 p2_ms += t;
+                if (debug_on())
+                    std::fprintf(stderr, "openph: m=%d iteration %d: Phase II %.3f ms\n", m, i + 1, t);
             }
     } else {
         // chunks of iterations; the first as long as the last reduction of
