@@ -260,6 +260,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="batched leg with the default setting instead of the autotuner's")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy, the reference kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -365,7 +367,7 @@ def main():
     }
     if not args.no_batched:
         print("bench: batched leg", file=sys.stderr, flush=True)
-        out["batched"] = batched_leg()
+        out["batched"] = batched_leg(tune=not args.no_tune)
     if not args.no_served:
         print("bench: served leg", file=sys.stderr, flush=True)
         out["served"] = served_leg()
