@@ -151,6 +151,32 @@ def batched_leg(count: int = 512, reps: int = 3, tune: bool = True):
     }
 
 
+def gpu_twist_leg(b, reps: int = 3):
+    """Torus-100 reduced by the twist algorithm on the same GPU (the
+    reference's column order, one workgroup taking the columns in turn) and
+    by pms, both through the host entry point (host COO in, host low/ess
+    out); outputs compared."""
+    import openph_amd
+
+    out = {}
+    res = {}
+    for algo in ("twist", "pms"):
+        openph_amd.reduce(b, algo)  # warm-up
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            low, ess, st = openph_amd.reduce(b, algo)
+            ts.append(time.perf_counter() - t0)
+        res[algo] = (low, ess)
+        out[f"{algo}_ms"] = statistics.median(ts) * 1e3
+        out[f"{algo}_cols_per_s"] = b.m / statistics.median(ts)
+        out[f"{algo}_additions"] = st.additions
+    out["pms_vs_twist"] = out["twist_ms"] / out["pms_ms"]
+    out["same_result"] = bool(np.array_equal(res["twist"][0], res["pms"][0])
+                              and np.array_equal(res["twist"][1], res["pms"][1]))
+    return out
+
+
 def served_leg(workers: int = 4, seed: int = 0):
     """Served pms (openph_server_*: worker streams, admission by pool
     footprint, continuous batching) against the plain drop-in (openph_v2,
@@ -260,6 +286,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-twist", action="store_true",
+                    help="skip the twist-on-the-same-GPU comparison")
     ap.add_argument("--no-tune", action="store_true",
                     help="batched leg with the default setting instead of the autotuner's")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -365,6 +393,9 @@ def main():
                 "pool_peak": st.pool_peak, "additions_per_s": st.additions * args.steps / wall,
                 "parity_vs_golden": ok},
     }
+    if not args.no_twist:
+        print("bench: twist on the same GPU", file=sys.stderr, flush=True)
+        out["gpu_twist"] = gpu_twist_leg(b)
     if not args.no_batched:
         print("bench: batched leg", file=sys.stderr, flush=True)
         out["batched"] = batched_leg(tune=not args.no_tune)

@@ -81,7 +81,10 @@ void openph(char *algstr, int *h_rows_in, int *h_cols_in, int m, int col_width,
 typedef struct openph_options {
     int device;              /* HIP device ordinal                           */
     int max_iters;           /* pms iteration cap (0 = 1,000,000)            */
-    int64_t pool_elems;      /* initial column-pool size in int32 entries
+    int64_t pool_elems;      /* initial column-pool size in int32 entries,
+                                counted from the initial layout (every
+                                column's entries, slot_elems at least): a
+                                smaller value leaves no headroom
                                 (0 = automatic)                              */
     int64_t pool_max_elems;  /* largest pool the reduction may grow to
                                 (0 = 2^32 - 1 entries, 16 GiB)               */

@@ -12,6 +12,7 @@ constexpr int kBlock = 256;        // threads per block for flat passes
 constexpr int kItems = 4;          // columns per thread in the tiled scans
 constexpr int kTile = kBlock * kItems;
 constexpr int kIntMax = 0x7fffffff;
+constexpr int kLongBlockLanes = 1024;  // lanes of the block kernels (long columns, serial)
 
 // Device-side scalars shared by all kernels of one reduction. Lives in
 // device memory; the host reads it back with one small copy.
@@ -91,6 +92,10 @@ struct Dev {
 
 // ---- launchers (kernels.hip) ----
 void launch_ctrl_init(const Dev &d, unsigned long long cap, int max_iters, hipStream_t s);
+// After the setup: pool capacity = max(requested, exact initial layout),
+// at most `alloc`.
+void launch_pool_cap(const Dev &d, unsigned long long requested, unsigned long long alloc,
+                     hipStream_t s);
 // COO -> per-column state and pool layout (slot-granular segments).
 void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t s);
 void launch_slots(const Dev &d, hipStream_t s);
@@ -121,6 +126,12 @@ bool phase2_shape_ok(int lanes, int wlds);
 int phase2_default_lds(int lanes);
 int phase2_blocks_per_cu(int lanes, int wlds);
 void launch_apply2(const Dev &d, hipStream_t s);
+// The serial algorithms (standard, twist): one workgroup reduces the columns
+// order[0..n) in turn. On a full pool it stops with Ctrl::overflow = 1 and
+// Ctrl::nwork = the position to resume from. stamps[k] gets the device clock
+// after column k, *start (when given) before the first.
+void launch_serial(const Dev &d, const int *order, int n, bool twist, unsigned long long *stamps,
+                   unsigned long long *start, hipStream_t s);
 void launch_final(const Dev &d, int *low1_out, int *ess_out, hipStream_t s);
 // ess_hat_final (kernels.cu:103-111) alone, and update_classes_final
 // (kernels.cu:79-91) on classes[]: the state the reference tracks after its

@@ -107,6 +107,10 @@ struct Context {
     // trackers: 0-based low_true on the device
     int *t_low = nullptr;
     size_t c_tlow = 0;
+    // serial algorithms: column order and per-column clock stamps
+    int *s_order = nullptr;
+    unsigned long long *s_stamps = nullptr;
+    size_t c_sorder = 0, c_sstamps = 0;
     bool init = false;
 
     Context() = default;
@@ -119,7 +123,7 @@ struct Context {
                         d.newpiv, d.slot, d.len, d.cap, d.work, d.longw, d.pivdesc, d.off, d.off2,
                         d.scan_tmp, d.slotmap, d.tile_max, d.tile_pre, d.pool, d.pool2, d.ctrl,
                         h_rows, h_cols, h_low, h_ess, b_moff, b_noff, b_rows, b_cols, b_err,
-                        t_low};
+                        t_low, s_order, s_stamps};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
         if (h_ctrl) (void)hipHostFree(h_ctrl);

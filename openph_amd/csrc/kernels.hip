@@ -220,6 +220,21 @@ void launch_ctrl_init(const Dev &d, unsigned long long cap, int max_iters, hipSt
     k_ctrl_init<<<1, 64, 0, s>>>(d.ctrl, cap, max_iters);
 }
 
+// A caller-given pool size counts from the exact initial layout, which the
+// setup scan leaves in pool_top: capacity = max(requested, layout), at most
+// the allocation.
+__global__ void k_pool_cap(Ctrl *c, unsigned long long requested, unsigned long long alloc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        unsigned long long cap = requested > c->pool_top ? requested : c->pool_top;
+        c->pool_cap = cap < alloc ? cap : alloc;
+    }
+}
+
+void launch_pool_cap(const Dev &d, unsigned long long requested, unsigned long long alloc,
+                     hipStream_t s) {
+    k_pool_cap<<<1, 64, 0, s>>>(d.ctrl, requested, alloc);
+}
+
 __device__ __forceinline__ int lower_bound_i(const int *a, int n, int key) {
     int lo = 0, hi = n;
     while (lo < hi) {
@@ -1118,7 +1133,7 @@ __global__ void __launch_bounds__(kBlock) k_phase2_group(Dev d) {
 // Columns handed off by the lane groups: one block of kLongBlock lanes per
 // column, 4096-entry LDS buffers, merging straight in the pool beyond that.
 constexpr int kLongW = 4096;
-constexpr int kLongBlock = 1024;
+constexpr int kLongBlock = kLongBlockLanes;
 
 __global__ void __launch_bounds__(kLongBlock) k_phase2_long(Dev d) {
     __shared__ int lds[3 * kLongW];
@@ -1196,6 +1211,163 @@ void launch_phase2(const Dev &d, int lanes, int wlds, int grid, int long_grid, h
     }
     k<<<grid, kBlock, 0, s>>>(d);
     k_phase2_long<<<long_grid, kLongBlock, 0, s>>>(d);
+}
+
+// ---------------------------------------------------------------------------
+// The serial algorithms on the device: standard (standard.cu:18-54) and
+// twist (twist.cu:16-87), whose GPU forms in the reference launch one
+// single-thread kernel per column (standard_parallel.cu, twist_parallel.cu).
+// Here one workgroup of kLongBlock lanes takes the columns of `order` one
+// after another. A column is reduced by the pivots of the columns before it
+// (reduce_col_host, kernels_host.cu:62-75): while its low has a pivot, the
+// pivot column is added (merge path over the block, in LDS while the sum
+// fits, in the pool beyond). It then becomes the pivot of its low; cell and
+// low are marked non-essential and their classes set (twist.cu:44-58); with
+// twist the low's own column is cleared (twist_step_host, twist.cu:1-7).
+// A full pool stops the kernel at the column it was on (Ctrl::nwork = its
+// position, Ctrl::overflow = 1); the host compacts and starts it again there.
+// stamps (may be null) receives the device clock after each column, start
+// (may be null) the clock before the first.
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kLongBlock) k_serial(Dev d, const int *__restrict__ order, int n,
+                                                       int twist, unsigned long long *stamps,
+                                                       unsigned long long *start) {
+    constexpr int W = kLongW;
+    __shared__ int lds[3 * W];
+    __shared__ int s_scan[kLongBlock / 64];
+    __shared__ unsigned long long s_word;
+    const BlockTeam<kLongBlock> t{s_scan, &s_word};
+    const int gl = threadIdx.x;
+    Phase2Counters ctr;
+    if (start && gl == 0) *start = (unsigned long long)wall_clock64();
+    for (int k = 0; k < n; k++) {
+        const int j = order[k];
+        int lj = d.low[j];
+        int4 pd = lj > -1 ? d.pivdesc[lj] : make_int4(-1, 0, 0, 0);
+        bool stop = false;
+        if (pd.x > -1) {
+            unsigned coff = d.off[j];
+            int clen = d.len[j], ccap = d.cap[j];
+            int *sA = lds, *sB = lds + W, *sC = lds + 2 * W;
+            bool glob = clen > W, dirty = false, lost = false;
+            if (!glob)
+                for (int i = gl; i < clen; i += kLongBlock) sB[i] = d.pool[coff + i];
+            t.sync();
+            while (pd.x > -1) {
+                const int na = pd.z;
+                const unsigned aoff = (unsigned)pd.y;
+                if (gl == 0) d.ess[lj] = 0;
+                int nc, nl;
+                if (!glob && na + clen <= W) {
+                    for (int i = gl; i < na; i += kLongBlock) sA[i] = d.pool[aoff + i];
+                    t.sync();
+                    nc = team_xor_lds(t, sA, na, sB, clen, sC);
+                    t.sync();
+                    int *tmp = sB;
+                    sB = sA;
+                    sA = tmp;
+                    nl = nc ? sB[nc - 1] : -1;
+                } else {
+                    const unsigned long long base = team_alloc(t, d, (unsigned)(na + clen));
+                    if (base == ~0ull) {
+                        ctr.ovf = true;
+                        lost = !glob && dirty;
+                        break;
+                    }
+                    const int *cur = glob ? d.pool + coff : sB;
+                    nc = team_xor(t, d.pool + aoff, na, cur, clen, d.pool + base);
+                    t.fence();
+                    if (!glob) ctr.spills++;
+                    glob = true;
+                    coff = (unsigned)base;
+                    ccap = na + clen;
+                    nl = nc ? d.pool[base + nc - 1] : -1;
+                }
+                ctr.nadd++;
+                ctr.nel += (unsigned long long)(na + clen + nc);
+                clen = nc;
+                dirty = true;
+                if (nl >= lj) {
+                    if (gl == 0) atomicOr(&d.ctrl->err, kErrState);
+                    break;
+                }
+                lj = nl;
+                if (lj < 0) break;
+                pd = d.pivdesc[lj];
+            }
+            if (ctr.ovf) {
+                // keep what is in the pool; the column is taken up again
+                // after compaction from the state committed here
+                if (!lost && dirty && glob && gl == 0) {
+                    d.off[j] = coff;
+                    d.len[j] = clen;
+                    d.cap[j] = ccap;
+                    d.low[j] = lj;
+                }
+                stop = true;
+            } else if (dirty) {
+                bool ok = true;
+                if (!glob) {
+                    if (clen > ccap) {
+                        unsigned want = (unsigned)clen;
+                        if (want < (unsigned)d.slot0) want = (unsigned)d.slot0;
+                        want = (want + 7u) & ~7u;
+                        const unsigned long long base = team_alloc(t, d, want);
+                        if (base == ~0ull) {
+                            ctr.ovf = true;
+                            ok = false;
+                        } else {
+                            coff = (unsigned)base;
+                            ccap = (int)want;
+                        }
+                    }
+                    if (ok)
+                        for (int i = gl; i < clen; i += kLongBlock) d.pool[coff + i] = sB[i];
+                }
+                if (ok && gl == 0) {
+                    d.off[j] = coff;
+                    d.len[j] = clen;
+                    d.cap[j] = ccap;
+                    d.low[j] = lj;
+                }
+                stop = !ok;
+            }
+        }
+        if (stop) {
+            if (gl == 0) {
+                d.ctrl->overflow = 1;
+                d.ctrl->nwork = k;
+            }
+            break;
+        }
+        if (gl == 0) {
+            if (lj > -1) {
+                publish_pivot(d, lj, j);
+                d.classes[j] = -1;
+                d.classes[lj] = 1;
+                d.ess[j] = 0;
+                d.ess[lj] = 0;
+                if (twist) d.low[lj] = -1;
+            } else {
+                d.classes[j] = 1;
+            }
+            if (stamps) stamps[k] = (unsigned long long)wall_clock64();
+        }
+        __syncthreads();  // the next column reads this one's pivot
+    }
+    if (gl == 0) {
+        if (ctr.nadd) {
+            atomicAdd(&d.ctrl->additions, ctr.nadd);
+            atomicAdd(&d.ctrl->merge_elems, ctr.nel);
+        }
+        if (ctr.spills) atomicAdd(&d.ctrl->spills, ctr.spills);
+    }
+}
+
+void launch_serial(const Dev &d, const int *order, int n, bool twist, unsigned long long *stamps,
+                   unsigned long long *start, hipStream_t s) {
+    k_serial<<<1, kLongBlock, 0, s>>>(d, order, n, twist ? 1 : 0, stamps, start);
 }
 
 // After Phase II: publish new pivots, clear_positives, ess_hat

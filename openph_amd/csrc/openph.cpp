@@ -377,9 +377,11 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
     size_t lcap = std::min(c.cap_pool, pool_want);
     launch_ctrl_init(d, (unsigned long long)lcap, cfg.max_iters, s);
     launch_setup(d, rows1, cols1, s);
+    if (cfg.pool_elems) launch_pool_cap(d, cfg.pool_elems, lcap, s);
     rc = read_ctrl(c, s);  // also completes the low_true upload from pageable memory
     if (rc) return rc;
     if ((rc = err_from_bits(c.h_ctrl->err))) return rc;
+    lcap = c.h_ctrl->pool_cap;
     const int maxlen = c.h_ctrl->maxlen;
     if (tracking && (rc = track_state(c, tr, 0, s, m, ess_true))) return rc;
     launch_slots(d, s);
@@ -590,6 +592,135 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
     return OPENPH_OK;
 }
 
+// standard and twist (and their _parallel names) run their own column
+// order on the device; the other algorithm strings run pms.
+static bool serial_algo(const char *a, bool *twist) {
+    if (!std::strcmp(a, "standard") || !std::strcmp(a, "standard_parallel")) {
+        *twist = false;
+        return true;
+    }
+    if (!std::strcmp(a, "twist") || !std::strcmp(a, "twist_parallel")) {
+        *twist = true;
+        return true;
+    }
+    return false;
+}
+
+// standard (standard.cu:18-54) and twist (twist.cu:16-87) on the device.
+// The column order comes from the column lengths (pseudo-dimension = length
+// - 1, dimensions.cu:54-104): standard takes every column in index order;
+// twist takes the pseudo-dimensions from the highest down to 1, each in index
+// order. num_iters is the reference's: one iteration per column taken, plus
+// one. time_track[i] is the device time of the i-th column; the err_*
+// trackers, which the reference evaluates over all m columns after every
+// column, are left as the caller set them.
+static int run_serial(Context &c, const int *rows1, const int *cols1, int m, int nnz, int *low1,
+                      int *ess, hipStream_t s, const RunCfg &cfg, bool twist, openph_stats *stats,
+                      const Track *tr) {
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (m == 0) {
+        if (stats) stats->num_iters = 1;
+        return OPENPH_OK;
+    }
+    const size_t layout = (size_t)nnz + (size_t)m * (size_t)cfg.slot0;
+    if (layout > cfg.pool_max) return OPENPH_ENOMEM;
+    size_t pool_want = cfg.pool_elems
+                           ? std::max(cfg.pool_elems, layout)
+                           : layout + std::max<size_t>((size_t)nnz * 8, (size_t)1 << 25);
+    pool_want = std::min(pool_want, cfg.pool_max);
+    int rc = ensure(c, m, nnz, pool_want);
+    if (rc) return rc;
+    Dev &d = c.d;
+    d.m = m;
+    d.nnz = nnz;
+    d.ntiles = (m + kTile - 1) / kTile;
+    d.slot0 = cfg.slot0;
+    if (!c.ev0) {
+        OPH_CHECK(hipEventCreate(&c.ev0));
+        OPH_CHECK(hipEventCreate(&c.ev1));
+    }
+    if (!c.h_ctrl) OPH_CHECK(hipHostMalloc((void **)&c.h_ctrl, sizeof(Ctrl), hipHostMallocDefault));
+    if (!c.h_word) OPH_CHECK(hipHostMalloc((void **)&c.h_word, 4 * sizeof(unsigned long long),
+                                           hipHostMallocDefault));
+    size_t lcap = std::min(c.cap_pool, pool_want);
+    launch_ctrl_init(d, (unsigned long long)lcap, cfg.max_iters, s);
+    launch_setup(d, rows1, cols1, s);
+    if (cfg.pool_elems) launch_pool_cap(d, cfg.pool_elems, lcap, s);
+    std::vector<int> len((size_t)m);
+    OPH_CHECK(hipMemcpyAsync(len.data(), d.len, sizeof(int) * m, hipMemcpyDeviceToHost, s));
+    if ((rc = read_ctrl(c, s))) return rc;
+    if ((rc = err_from_bits(c.h_ctrl->err))) return rc;
+    lcap = c.h_ctrl->pool_cap;
+    std::vector<int> order;
+    order.reserve((size_t)m);
+    if (!twist) {
+        for (int j = 0; j < m; j++) order.push_back(j);
+    } else {
+        const int maxlen = c.h_ctrl->maxlen;
+        std::vector<int> count((size_t)maxlen + 2, 0), at((size_t)maxlen + 2, 0);
+        for (int j = 0; j < m; j++) count[len[j]]++;
+        int o = 0;
+        for (int L = maxlen; L >= 2; L--) {
+            at[L] = o;
+            o += count[L];
+        }
+        order.resize((size_t)o);
+        for (int j = 0; j < m; j++)
+            if (len[j] >= 2) order[at[len[j]]++] = j;
+    }
+    const int n = (int)order.size();
+    OPH_CHECK(grow(&c.s_order, c.c_sorder, (size_t)n + 1));
+    if (n) OPH_CHECK(hipMemcpyAsync(c.s_order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    const bool times = tr && tr->time_track;
+    if (times) OPH_CHECK(grow(&c.s_stamps, c.c_sstamps, (size_t)n + 1));
+    OPH_CHECK(hipEventRecord(c.ev0, s));
+    int k0 = 0, ncompact = 0, stalls = 0, launches = 0;
+    const Ctrl &h = *c.h_ctrl;
+    while (k0 < n) {
+        launch_serial(d, c.s_order + k0, n - k0, twist, times ? c.s_stamps + 1 + k0 : nullptr,
+                      times && k0 == 0 ? c.s_stamps : nullptr, s);
+        launches++;
+        if ((rc = read_ctrl(c, s))) return rc;
+        if (h.err) return err_from_bits(h.err) ? err_from_bits(h.err) : OPENPH_EINVAL;
+        if (!h.overflow) break;
+        // a full pool: compact, then take the same column up again
+        stalls = h.nwork == 0 ? stalls + 1 : 0;
+        if (stalls >= 2) return OPENPH_ENOMEM;
+        k0 += h.nwork;
+        if ((rc = compact(c, cfg, s, &ncompact, lcap))) return rc;
+        launch_iter_begin(d, s);  // clears the overflow flag
+    }
+    launch_final(d, low1, ess, s);
+    OPH_CHECK(hipEventRecord(c.ev1, s));
+    if ((rc = read_ctrl(c, s))) return rc;
+    if (stats) {
+        stats->num_iters = n + 1;
+        stats->compactions = ncompact;
+        stats->overflows = launches - 1;
+        stats->additions = (int64_t)h.additions;
+        stats->merge_elems = (int64_t)h.merge_elems;
+        stats->pool_peak = (int64_t)std::max<unsigned long long>(h.pool_top, (unsigned long long)nnz);
+        float ms = 0.f;
+        OPH_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+        stats->device_ms = ms;
+        stats->m = m;
+        stats->lanes = kLongBlockLanes;
+        stats->spills = h.spills;
+        stats->host_reads = launches;
+    }
+    if (times) {
+        int khz = 0;
+        OPH_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c.device));
+        std::vector<unsigned long long> st((size_t)n + 1);
+        OPH_CHECK(hipMemcpy(st.data(), c.s_stamps, sizeof(unsigned long long) * (n + 1),
+                            hipMemcpyDeviceToHost));
+        tr->time_track[0] = 0.f;
+        for (int k = 1; k <= n && k <= m; k++)
+            tr->time_track[k] = khz > 0 && st[k] >= st[k - 1] ? (float)((double)(st[k] - st[k - 1]) / khz) : 0.f;
+    }
+    return OPENPH_OK;
+}
+
 static bool known_algo(const char *a) {
     static const char *names[] = {"pms", "standard", "twist", "ph_row",
                                   "standard_parallel", "twist_parallel", "ph_row_parallel"};
@@ -630,7 +761,12 @@ static int run_host(const char *algstr, const int *rows1, const int *cols1, int 
         OPH_CHECK(hipMemcpyAsync(c->h_cols, cols1, sizeof(int) * nnz, hipMemcpyHostToDevice, s));
     }
     RunCfg cfg = cfg_from(opt, nnz);
-    rc = run_pms(*c, c->h_rows, c->h_cols, m, nnz, c->h_low, c->h_ess, s, cfg, stats, tr);
+    bool twist = false;
+    if (serial_algo(algstr, &twist))
+        rc = run_serial(*c, c->h_rows, c->h_cols, m, nnz, c->h_low, c->h_ess, s, cfg, twist, stats,
+                        tr);
+    else
+        rc = run_pms(*c, c->h_rows, c->h_cols, m, nnz, c->h_low, c->h_ess, s, cfg, stats, tr);
     if (rc) {
         (void)hipStreamSynchronize(s);
         return rc;

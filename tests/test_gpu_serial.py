@@ -1,0 +1,96 @@
+"""GPU tests of the serial algorithms on the device: standard
+(src/cuda/algorithms/standard/standard.cu:18-54) and twist
+(src/cuda/algorithms/twist/twist.cu:16-87), also under their _parallel
+names. They run the reference's own column order, so besides low/ess the
+number of column additions, the merged elements and num_iters (columns taken
++ 1) must equal the oracle's restatement of the same algorithm."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import openph_amd
+from openph_amd import complexes as C
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+INDEX = json.load(open(os.path.join(GOLD, "index.json")))
+sys.path.insert(0, os.path.dirname(__file__))
+from test_oracle import _gen  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    openph_amd.lib()
+
+
+@pytest.mark.parametrize("algo", ["standard", "twist", "standard_parallel", "twist_parallel"])
+@pytest.mark.parametrize("name", ["vr12", "random0", "random3", "vr20_sphere", "vr40", "cubical8",
+                                  "cubical16"])
+def test_serial_algorithms_match_oracle(name, algo):
+    say("serial", name, algo)
+    f = np.load(os.path.join(GOLD, name + ".npz"))
+    b = _gen(name)
+    low, ess, st = openph_amd.reduce(b, algo)
+    assert np.array_equal(low, f["low"]) and np.array_equal(ess, f["ess"])
+    ref = oracle.twist(b) if algo.startswith("twist") else oracle.standard(b)
+    assert st.additions == ref.additions
+    assert st.merge_elems == ref.merge_elems
+    assert st.num_iters == ref.iters + 1
+
+
+def test_twist_on_torus_and_trackers_time_each_column():
+    say("twist torus")
+    f = np.load(os.path.join(GOLD, "torus100.npz"))
+    b = _gen("torus100")
+    r, c = b.coo1()
+    out = openph_amd.ph("twist", r, c, b.m, 8, None)
+    n = int(out[7][0])
+    ref = oracle.twist(b)
+    assert n == ref.iters + 1
+    assert np.array_equal(out[0], f["low"] + 1) and np.array_equal(out[1], f["ess"])
+    tt = out[6]
+    assert tt[0] == 0 and (tt[1:n] >= 0).all() and tt[1:n].sum() > 0 and (tt[n:] == 0).all()
+    # the err_* trackers are left as initialised
+    assert (out[2] == -1).all() and (out[3] == -1).all()
+
+
+@pytest.mark.parametrize("algo", ["standard", "twist"])
+def test_serial_small_pool_resumes_exactly(algo):
+    say("serial small pool", algo)
+    b = _gen("torus100")
+    f = np.load(os.path.join(GOLD, "torus100.npz"))
+    opt = openph_amd.Options()
+    opt.slot_elems = 1
+    # exactly the initial layout (one entry at least per column)
+    opt.pool_elems = int(np.maximum(np.diff(b.colptr), 1).sum())
+    low, ess, st = openph_amd.reduce(b, algo, opt)
+    assert st.compactions > 0
+    assert np.array_equal(low, f["low"]) and np.array_equal(ess, f["ess"])
+    ref = oracle.twist(b) if algo == "twist" else oracle.standard(b)
+    assert st.num_iters == ref.iters + 1
+
+
+@pytest.mark.parametrize("algo", ["standard", "twist"])
+def test_serial_edge_cases(algo):
+    say("serial edge", algo)
+    b = C.Boundary(colptr=np.zeros(6, np.int64), rows=np.zeros(0, np.int32),
+                   dim=np.zeros(5, np.int32), value=np.zeros(5))
+    low, ess, st = openph_amd.reduce(b, algo)
+    assert (low == -1).all() and (ess == 1).all()
+    assert st.num_iters == (6 if algo == "standard" else 1)
+    b = C.Boundary(colptr=np.zeros(1, np.int64), rows=np.zeros(0, np.int32),
+                   dim=np.zeros(0, np.int32), value=np.zeros(0))
+    low, ess, st = openph_amd.reduce(b, algo)
+    assert low.size == 0 and st.num_iters == 1
