@@ -249,6 +249,35 @@ int openph_server_stats(openph_server *srv, int64_t *queued, int64_t *batches,
 /* Finishes queued requests, then stops the workers and frees the server. */
 int openph_server_destroy(openph_server *srv);
 
+/*
+ * Filtrations built on the device (no counterpart in the reference, which
+ * builds its matrices in Javaplex on the host,
+ * numerics/utils/pointcloud_to_vr_stream.m, numerics/utils/stream2cmo.m:1-26).
+ * They write the 1-based COO boundary matrix in filtration order, ready for
+ * openph_pms_device, plus each column's filtration value and dimension
+ * (values and dims may be NULL). All array arguments are device pointers on
+ * `device`; the call returns when the outputs are written.
+ *
+ * VR 2-skeleton at full radius of n points in R^dim (dim <= 7, row-major
+ * doubles): vertices, edges and triangles ordered by value (0, length,
+ * longest edge), then dimension, then vertex tuple. Lengths are computed as
+ * sqrt(sum_k (p_ik - p_jk)^2), summed from k = 0 with every operation
+ * rounded once. m = n + C(n,2) + C(n,3), nnz = 2 C(n,2) + 3 C(n,3).
+ */
+int openph_vr2_size(int n, int64_t *m, int64_t *nnz);
+int openph_vr2_device(const double *points, int n, int dim, int *rows1, int *cols1,
+                      double *values, int *dims, void *stream, int device);
+
+/*
+ * Lower-star filtration of the cubical complex of an s0 x s1 x s2 grid of
+ * float values (row-major): all (2 s0 - 1)(2 s1 - 1)(2 s2 - 1) cells, value
+ * of a cell = max over its vertices, ordered by value, then dimension, then
+ * linear cell index. m must stay below 2^28.
+ */
+int openph_cubical_size(int s0, int s1, int s2, int64_t *m, int64_t *nnz);
+int openph_cubical_device(const float *field, int s0, int s1, int s2, int *rows1, int *cols1,
+                          float *values, int *dims, void *stream, int device);
+
 /* Human-readable message for a status code. */
 const char *openph_strerror(int status);
 

@@ -26,7 +26,7 @@ import numpy as np
 __all__ = [
     "OpenPHError", "Options", "Stats", "lib", "build", "ph", "openph", "reduce",
     "pms_device", "pms_batch", "pms_batch_device", "autotune_batch_device", "TuneRow",
-    "ALGORITHMS",
+    "vr2_device", "cubical_device", "ALGORITHMS",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -150,6 +150,20 @@ def lib():
             ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Options), ctypes.c_int,
             ctypes.POINTER(Options), ctypes.POINTER(TuneRow), ctypes.c_int,
             ctypes.POINTER(ctypes.c_int)]
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        L.openph_vr2_size.restype = ctypes.c_int
+        L.openph_vr2_size.argtypes = [ctypes.c_int, i64p, i64p]
+        L.openph_vr2_device.restype = ctypes.c_int
+        L.openph_vr2_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        L.openph_cubical_size.restype = ctypes.c_int
+        L.openph_cubical_size.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, i64p, i64p]
+        L.openph_cubical_device.restype = ctypes.c_int
+        L.openph_cubical_device.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_int]
         L.openph_strerror.restype = ctypes.c_char_p
         L.openph_strerror.argtypes = [ctypes.c_int]
         L.openph_version.restype = ctypes.c_char_p
@@ -318,3 +332,56 @@ def autotune_batch_device(ms, nnzs, rows1, cols1, low1, ess, stream=None,
     if rc != 0:
         raise OpenPHError(rc, "autotune_batch_device")
     return best, [rows[i].as_dict() for i in range(n.value)]
+
+
+def _stream_ptr(stream):
+    return stream.cuda_stream if stream is not None and hasattr(stream, "cuda_stream") else stream
+
+
+def vr2_device(points, stream=None, device: int = 0):
+    """Full-radius VR 2-skeleton of an (n, d) float64 CUDA tensor of points
+    (d <= 7), built on the device. Returns (rows1, cols1, values, dims) as
+    device tensors: the 1-based COO in filtration order, each column's value
+    and dimension (the same matrix as complexes.vr_2skeleton)."""
+    import torch
+
+    n, d = points.shape
+    m = ctypes.c_int64()
+    nnz = ctypes.c_int64()
+    lib().openph_vr2_size(n, ctypes.byref(m), ctypes.byref(nnz))
+    dev = points.device
+    rows = torch.empty(max(nnz.value, 1), dtype=torch.int32, device=dev)
+    cols = torch.empty_like(rows)
+    vals = torch.empty(max(m.value, 1), dtype=torch.float64, device=dev)
+    dims = torch.empty(max(m.value, 1), dtype=torch.int32, device=dev)
+    pts = points.contiguous()
+    rc = lib().openph_vr2_device(pts.data_ptr(), n, d, rows.data_ptr(), cols.data_ptr(),
+                                 vals.data_ptr(), dims.data_ptr(), _stream_ptr(stream), device)
+    if rc != 0:
+        raise OpenPHError(rc, "vr2_device")
+    return rows[:nnz.value], cols[:nnz.value], vals[:m.value], dims[:m.value]
+
+
+def cubical_device(field, stream=None, device: int = 0):
+    """Lower-star filtration of the cubical complex of an (s0, s1, s2)
+    float32 CUDA tensor, built on the device. Returns (rows1, cols1, values,
+    dims) as device tensors (the same matrix as complexes.cubical_lower_star)."""
+    import torch
+
+    s0, s1, s2 = field.shape
+    m = ctypes.c_int64()
+    nnz = ctypes.c_int64()
+    rc = lib().openph_cubical_size(s0, s1, s2, ctypes.byref(m), ctypes.byref(nnz))
+    if rc != 0:
+        raise OpenPHError(rc, "cubical_device")
+    dev = field.device
+    rows = torch.empty(max(nnz.value, 1), dtype=torch.int32, device=dev)
+    cols = torch.empty_like(rows)
+    vals = torch.empty(max(m.value, 1), dtype=torch.float32, device=dev)
+    dims = torch.empty(max(m.value, 1), dtype=torch.int32, device=dev)
+    f = field.contiguous()
+    rc = lib().openph_cubical_device(f.data_ptr(), s0, s1, s2, rows.data_ptr(), cols.data_ptr(),
+                                     vals.data_ptr(), dims.data_ptr(), _stream_ptr(stream), device)
+    if rc != 0:
+        raise OpenPHError(rc, "cubical_device")
+    return rows[:nnz.value], cols[:nnz.value], vals[:m.value], dims[:m.value]

@@ -111,6 +111,15 @@ struct Context {
     int *s_order = nullptr;
     unsigned long long *s_stamps = nullptr;
     size_t c_sorder = 0, c_sstamps = 0;
+    // filtration builders: sort keys and values (in, out), positions,
+    // column offsets, pairwise distances and sort scratch
+    unsigned long long *f_key = nullptr, *f_key2 = nullptr;
+    unsigned *f_val = nullptr, *f_val2 = nullptr;
+    int *f_pos = nullptr, *f_nnz = nullptr, *f_off = nullptr;
+    double *f_dist = nullptr;
+    unsigned char *f_tmp = nullptr;
+    size_t c_fkey = 0, c_fkey2 = 0, c_fval = 0, c_fval2 = 0, c_fpos = 0, c_fnnz = 0, c_foff = 0,
+           c_fdist = 0, c_ftmp = 0;
     bool init = false;
 
     Context() = default;
@@ -123,7 +132,8 @@ struct Context {
                         d.newpiv, d.slot, d.len, d.cap, d.work, d.longw, d.pivdesc, d.off, d.off2,
                         d.scan_tmp, d.slotmap, d.tile_max, d.tile_pre, d.pool, d.pool2, d.ctrl,
                         h_rows, h_cols, h_low, h_ess, b_moff, b_noff, b_rows, b_cols, b_err,
-                        t_low, s_order, s_stamps};
+                        t_low, s_order, s_stamps, f_key, f_key2, f_val, f_val2, f_pos, f_nnz,
+                        f_off, f_dist, f_tmp};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
         if (h_ctrl) (void)hipHostFree(h_ctrl);

@@ -106,3 +106,21 @@ def test_autotune_argument_checks_on_host():
                                           None, None, 0, None) == -1
     assert L.openph_autotune_batch_device(1, ip, ip, None, None, None, None, None, None, 1,
                                           ctypes.byref(best), None, -1, None) == -1
+
+
+def test_filtration_sizes_on_host():
+    import math
+    L = openph_amd.lib()
+    m = ctypes.c_int64()
+    nnz = ctypes.c_int64()
+    for n in (0, 1, 2, 3, 40, 200):
+        assert L.openph_vr2_size(n, ctypes.byref(m), ctypes.byref(nnz)) == 0
+        assert m.value == n + math.comb(n, 2) + math.comb(n, 3)
+        assert nnz.value == 2 * math.comb(n, 2) + 3 * math.comb(n, 3)
+    from openph_amd import complexes as C
+    for shape in ((1, 1, 1), (2, 3, 4), (8, 8, 8)):
+        assert L.openph_cubical_size(*shape, ctypes.byref(m), ctypes.byref(nnz)) == 0
+        b = C.cubical_lower_star(np.zeros(shape, np.float32))
+        assert m.value == b.m and nnz.value == b.nnz
+    assert L.openph_cubical_size(0, 2, 2, ctypes.byref(m), ctypes.byref(nnz)) == -1
+    assert L.openph_vr2_size(-1, ctypes.byref(m), ctypes.byref(nnz)) == -1
