@@ -180,6 +180,43 @@ def gpu_twist_leg(b, reps: int = 3):
     return out
 
 
+def builder_leg(reps: int = 5):
+    """The boundary matrices built on the device (openph_vr2_device,
+    openph_cubical_device) against the host builder (openph_amd.complexes),
+    from the same point cloud / field; outputs compared."""
+    import torch
+
+    import openph_amd
+    from openph_amd import complexes as C
+
+    out = {}
+    rng = np.random.default_rng(0)
+    pts = C.points_torus(rng, 100)
+    field = C.smoothed_field(np.random.default_rng(0), 64)
+    for name, host, dev_in, build in (
+            ("vr_torus100", lambda: C.vr_2skeleton(pts), torch.from_numpy(pts).cuda(),
+             openph_amd.vr2_device),
+            ("cubical64", lambda: C.cubical_lower_star(field), torch.from_numpy(field).cuda(),
+             openph_amd.cubical_device)):
+        t0 = time.perf_counter()
+        b = host()
+        t_host = time.perf_counter() - t0
+        res = build(dev_in)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            res = build(dev_in)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        r, c = b.coo1()
+        same = bool(np.array_equal(res[0].cpu().numpy(), r) and np.array_equal(res[1].cpu().numpy(), c))
+        out[name] = {"m": b.m, "nnz": b.nnz, "device_ms": statistics.median(ts) * 1e3,
+                     "host_ms": t_host * 1e3, "speedup": t_host / statistics.median(ts),
+                     "same_matrix": same}
+    return out
+
+
 def served_leg(workers: int = 4, seed: int = 0):
     """Served pms (openph_server_*: worker streams, admission by pool
     footprint, continuous batching) against the plain drop-in (openph_v2,
@@ -289,6 +326,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-builders", action="store_true",
+                    help="skip the device-built filtration comparison")
     ap.add_argument("--no-twist", action="store_true",
                     help="skip the twist-on-the-same-GPU comparison")
     ap.add_argument("--no-tune", action="store_true",
@@ -399,6 +438,9 @@ def main():
     if not args.no_twist:
         print("bench: twist on the same GPU", file=sys.stderr, flush=True)
         out["gpu_twist"] = gpu_twist_leg(b)
+    if not args.no_builders:
+        print("bench: builders", file=sys.stderr, flush=True)
+        out["builders"] = builder_leg()
     if not args.no_batched:
         print("bench: batched leg", file=sys.stderr, flush=True)
         out["batched"] = batched_leg(tune=not args.no_tune)
