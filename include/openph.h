@@ -240,6 +240,15 @@ int openph_server_create(const openph_server_options *opt, openph_server **out);
  * for the device; OPENPH_EBUSY when the queue is full. */
 int openph_server_submit(openph_server *srv, const int *rows1, const int *cols1, int m, int nnz,
                          int64_t *ticket);
+/* Queue a matrix the worker builds on the device from its input (host
+ * arrays, copied): the VR 2-skeleton of n points in R^dim, or the cubical
+ * lower-star filtration of an s0 x s1 x s2 grid (see openph_vr2_device and
+ * openph_cubical_device). Non-finite points and NaN field values are
+ * refused. */
+int openph_server_submit_vr(openph_server *srv, const double *points, int n, int dim,
+                            int64_t *ticket);
+int openph_server_submit_cubical(openph_server *srv, const float *field, int s0, int s1, int s2,
+                                 int64_t *ticket);
 /* Block until the ticket's reduction is done; low1/ess (host, m entries) get
  * its outputs. A ticket can be waited for once. */
 int openph_server_wait(openph_server *srv, int64_t ticket, int *low1, int *ess,

@@ -279,33 +279,15 @@ int scan_offsets(Context &c, const int *nnz, int *off, long long m, hipStream_t 
 
 }  // namespace
 
-}  // namespace oph
-
-using namespace oph;
-
-extern "C" {
-
-int openph_vr2_size(int n, int64_t *m, int64_t *nnz) {
-    if (n < 0) return OPENPH_EINVAL;
-    const long long E = choose2(n), T = choose3(n);
-    if (m) *m = n + E + T;
-    if (nnz) *nnz = 2 * E + 3 * T;
-    return OPENPH_OK;
-}
-
-int openph_vr2_device(const double *points, int n, int dim, int *rows1, int *cols1, double *values,
-                      int *dims, void *stream, int device) {
-    if (n < 0 || dim < 1 || dim > 7 || (n > 0 && !points)) return OPENPH_EINVAL;
+// The two builders on a given workspace and stream, without waiting for
+// them (the server's workers build requests inside their batches).
+int build_vr(Context &c, const double *points, int n, int dim, int *rows1, int *cols1,
+             double *values, int *dims, hipStream_t s) {
     const long long E = choose2(n), T = choose3(n), m = n + E + T, nnz = 2 * E + 3 * T;
+    if (n < 0 || dim < 1 || dim > 7 || (n > 0 && !points)) return OPENPH_EINVAL;
     if (m > 0x7fffffffLL || nnz > 0x7fffffffLL) return OPENPH_EINVAL;
     if (nnz > 0 && (!rows1 || !cols1)) return OPENPH_EINVAL;
     if (m == 0) return OPENPH_OK;
-    int rc = set_device(device);
-    if (rc) return rc;
-    Context *cp = context_for(device);
-    std::lock_guard<std::mutex> g(cp->mu);
-    Context &c = *cp;
-    hipStream_t s = (hipStream_t)stream;
     OPH_CHECK(grow(&c.f_dist, c.c_fdist, (size_t)n * n));
     OPH_CHECK(grow(&c.f_key, c.c_fkey, (size_t)m));
     OPH_CHECK(grow(&c.f_key2, c.c_fkey2, (size_t)m));
@@ -324,10 +306,67 @@ int openph_vr2_device(const double *points, int n, int dim, int *rows1, int *col
                                                  c.f_val2, (int)m, 0, 64, s));
     k_scatter_pos<<<grid_for(m), kBlock, 0, s>>>(c.f_val2, m, c.f_pos);
     k_vr_nnz<<<grid_for(m), kBlock, 0, s>>>(c.f_val2, n, E, m, c.f_nnz);
-    if ((rc = scan_offsets(c, c.f_nnz, c.f_off, m, s))) return rc;
+    int rc = scan_offsets(c, c.f_nnz, c.f_off, m, s);
+    if (rc) return rc;
     k_vr_coo<<<grid_for(m), kBlock, 0, s>>>(c.f_val2, c.f_key2, c.f_pos, c.f_off, n, E, m, rows1,
                                              cols1, values, dims);
     OPH_CHECK(hipGetLastError());
+    return OPENPH_OK;
+}
+
+int build_cubical(Context &c, const float *field, int s0, int s1, int s2, int *rows1, int *cols1,
+                  float *values, int *dims, hipStream_t s) {
+    int64_t m64 = 0, nnz64 = 0;
+    if (s0 < 1 || s1 < 1 || s2 < 1 || !field) return OPENPH_EINVAL;
+    const long long N0 = 2LL * s0 - 1, N1 = 2LL * s1 - 1, N2 = 2LL * s2 - 1;
+    m64 = N0 * N1 * N2;
+    nnz64 = 2 * ((s0 - 1) * N1 * N2 + N0 * (s1 - 1) * N2 + N0 * N1 * (s2 - 1));
+    if (m64 >= (1LL << kCubIndexBits) || nnz64 > 0x7fffffffLL) return OPENPH_EINVAL;
+    if (nnz64 > 0 && (!rows1 || !cols1)) return OPENPH_EINVAL;
+    const long long m = m64;
+    OPH_CHECK(grow(&c.f_key, c.c_fkey, (size_t)m));
+    OPH_CHECK(grow(&c.f_key2, c.c_fkey2, (size_t)m));
+    OPH_CHECK(grow(&c.f_pos, c.c_fpos, (size_t)m));
+    OPH_CHECK(grow(&c.f_nnz, c.c_fnnz, (size_t)m));
+    OPH_CHECK(grow(&c.f_off, c.c_foff, (size_t)m));
+    k_cub_keys<<<grid_for(m), kBlock, 0, s>>>(field, s0, s1, s2, m, c.f_key);
+    size_t bytes = 0;
+    OPH_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, c.f_key, c.f_key2, (int)m, 0, 64, s));
+    OPH_CHECK(grow(&c.f_tmp, c.c_ftmp, bytes + 1));
+    OPH_CHECK(hipcub::DeviceRadixSort::SortKeys(c.f_tmp, bytes, c.f_key, c.f_key2, (int)m, 0, 64, s));
+    k_cub_pos<<<grid_for(m), kBlock, 0, s>>>(c.f_key2, m, c.f_pos, c.f_nnz);
+    int rc = scan_offsets(c, c.f_nnz, c.f_off, m, s);
+    if (rc) return rc;
+    k_cub_coo<<<grid_for(m), kBlock, 0, s>>>(c.f_key2, field, s0, s1, s2, m, c.f_pos, c.f_off,
+                                              rows1, cols1, values, dims);
+    OPH_CHECK(hipGetLastError());
+    return OPENPH_OK;
+}
+
+}  // namespace oph
+
+using namespace oph;
+
+extern "C" {
+
+int openph_vr2_size(int n, int64_t *m, int64_t *nnz) {
+    if (n < 0) return OPENPH_EINVAL;
+    const long long E = choose2(n), T = choose3(n);
+    if (m) *m = n + E + T;
+    if (nnz) *nnz = 2 * E + 3 * T;
+    return OPENPH_OK;
+}
+
+int openph_vr2_device(const double *points, int n, int dim, int *rows1, int *cols1, double *values,
+                      int *dims, void *stream, int device) {
+    if (n < 0 || dim < 1 || dim > 7) return OPENPH_EINVAL;
+    if (n == 0) return OPENPH_OK;
+    int rc = set_device(device);
+    if (rc) return rc;
+    Context *cp = context_for(device);
+    std::lock_guard<std::mutex> g(cp->mu);
+    hipStream_t s = (hipStream_t)stream;
+    if ((rc = build_vr(*cp, points, n, dim, rows1, cols1, values, dims, s))) return rc;
     OPH_CHECK(hipStreamSynchronize(s));
     return OPENPH_OK;
 }
@@ -343,32 +382,13 @@ int openph_cubical_size(int s0, int s1, int s2, int64_t *m, int64_t *nnz) {
 
 int openph_cubical_device(const float *field, int s0, int s1, int s2, int *rows1, int *cols1,
                           float *values, int *dims, void *stream, int device) {
-    int64_t m64 = 0, nnz64 = 0;
-    int rc = openph_cubical_size(s0, s1, s2, &m64, &nnz64);
+    int rc = openph_cubical_size(s0, s1, s2, nullptr, nullptr);
     if (rc) return rc;
-    if (m64 >= (1LL << kCubIndexBits) || nnz64 > 0x7fffffffLL || !field) return OPENPH_EINVAL;
-    if (nnz64 > 0 && (!rows1 || !cols1)) return OPENPH_EINVAL;
-    const long long m = m64;
     if ((rc = set_device(device))) return rc;
     Context *cp = context_for(device);
     std::lock_guard<std::mutex> g(cp->mu);
-    Context &c = *cp;
     hipStream_t s = (hipStream_t)stream;
-    OPH_CHECK(grow(&c.f_key, c.c_fkey, (size_t)m));
-    OPH_CHECK(grow(&c.f_key2, c.c_fkey2, (size_t)m));
-    OPH_CHECK(grow(&c.f_pos, c.c_fpos, (size_t)m));
-    OPH_CHECK(grow(&c.f_nnz, c.c_fnnz, (size_t)m));
-    OPH_CHECK(grow(&c.f_off, c.c_foff, (size_t)m));
-    k_cub_keys<<<grid_for(m), kBlock, 0, s>>>(field, s0, s1, s2, m, c.f_key);
-    size_t bytes = 0;
-    OPH_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, c.f_key, c.f_key2, (int)m, 0, 64, s));
-    OPH_CHECK(grow(&c.f_tmp, c.c_ftmp, bytes + 1));
-    OPH_CHECK(hipcub::DeviceRadixSort::SortKeys(c.f_tmp, bytes, c.f_key, c.f_key2, (int)m, 0, 64, s));
-    k_cub_pos<<<grid_for(m), kBlock, 0, s>>>(c.f_key2, m, c.f_pos, c.f_nnz);
-    if ((rc = scan_offsets(c, c.f_nnz, c.f_off, m, s))) return rc;
-    k_cub_coo<<<grid_for(m), kBlock, 0, s>>>(c.f_key2, field, s0, s1, s2, m, c.f_pos, c.f_off,
-                                              rows1, cols1, values, dims);
-    OPH_CHECK(hipGetLastError());
+    if ((rc = build_cubical(*cp, field, s0, s1, s2, rows1, cols1, values, dims, s))) return rc;
     OPH_CHECK(hipStreamSynchronize(s));
     return OPENPH_OK;
 }

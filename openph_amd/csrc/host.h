@@ -186,6 +186,12 @@ struct RunCfg {
 RunCfg cfg_from(const openph_options *opt, int nnz);
 // The workspace of a device, created on first use and kept for the process.
 Context *context_for(int device);
+// Filtrations built on the device into rows1/cols1 (1-based COO), queued on
+// `s` without waiting (filtrations.hip).
+int build_vr(Context &c, const double *points, int n, int dim, int *rows1, int *cols1,
+             double *values, int *dims, hipStream_t s);
+int build_cubical(Context &c, const float *field, int s0, int s1, int s2, int *rows1, int *cols1,
+                  float *values, int *dims, hipStream_t s);
 // Sizes a workspace for batches of up to m columns, nnz entries and a pool of
 // pool_elems entries, so that serving does not allocate (hipFree waits for
 // the whole device).

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -42,7 +43,14 @@ static double now_s() {
 struct Request {
     int64_t id = 0;
     int m = 0, nnz = 0;
+    // the matrix: given as 1-based COO (kind 0), or built on the worker's
+    // stream from a point cloud (kind 1, VR 2-skeleton) or a grid (kind 2,
+    // cubical lower-star)
+    int kind = 0;
     std::vector<int> rows, cols;
+    std::vector<double> points;
+    std::vector<float> field;
+    int n = 0, pdim = 0, s0 = 0, s1 = 0, s2 = 0;
     std::vector<int> low, ess;
     size_t footprint = 0;
     double t_submit = 0, t_start = 0, t_done = 0;
@@ -59,14 +67,19 @@ struct Worker {
     size_t c_rows = 0, c_cols = 0, c_low = 0, c_ess = 0;
     int *d_low = nullptr, *d_ess = nullptr;
     size_t c_dlow = 0, c_dess = 0;
+    // inputs of the requests built on the device: pinned staging, device copy
+    unsigned char *h_in = nullptr, *d_in = nullptr;
+    size_t c_hin = 0, c_din = 0;
     std::thread thread;
 
     ~Worker() {
         int *pinned[] = {h_rows, h_cols, h_low, h_ess};
         for (int *p : pinned)
             if (p) (void)hipHostFree(p);
+        if (h_in) (void)hipHostFree(h_in);
         if (d_low) (void)hipFree(d_low);
         if (d_ess) (void)hipFree(d_ess);
+        if (d_in) (void)hipFree(d_in);
     }
 };
 
@@ -134,18 +147,56 @@ static int run_worker_batch(openph_server *srv, Worker &w, std::vector<Request *
     OPH_CHECK(grow(&c.h_cols, c.c_hcols, N + 1));
     OPH_CHECK(grow(&w.d_low, w.c_dlow, M + 1));
     OPH_CHECK(grow(&w.d_ess, w.c_dess, M + 1));
-    size_t o = 0;
+    size_t o = 0, in_bytes = 0;
     for (Request *r : batch) {
-        if (r->nnz) {
+        if (r->kind == 0 && r->nnz) {
             std::memcpy(w.h_rows + o, r->rows.data(), sizeof(int) * r->nnz);
             std::memcpy(w.h_cols + o, r->cols.data(), sizeof(int) * r->nnz);
         }
+        if (r->kind == 1) in_bytes += ((r->points.size() * sizeof(double)) + 255) & ~(size_t)255;
+        if (r->kind == 2) in_bytes += ((r->field.size() * sizeof(float)) + 255) & ~(size_t)255;
         o += r->nnz;
     }
     hipStream_t s = w.stream;
+    // the given COO goes in as one copy; the built matrices are then written
+    // over their own ranges on the same stream
     if (N) {
         OPH_CHECK(hipMemcpyAsync(c.h_rows, w.h_rows, sizeof(int) * N, hipMemcpyHostToDevice, s));
         OPH_CHECK(hipMemcpyAsync(c.h_cols, w.h_cols, sizeof(int) * N, hipMemcpyHostToDevice, s));
+    }
+    if (in_bytes) {
+        OPH_CHECK(grow_pinned(&w.h_in, w.c_hin, in_bytes));
+        OPH_CHECK(grow(&w.d_in, w.c_din, in_bytes));
+        size_t b = 0;
+        for (Request *r : batch) {
+            if (r->kind == 1) {
+                std::memcpy(w.h_in + b, r->points.data(), r->points.size() * sizeof(double));
+                b += ((r->points.size() * sizeof(double)) + 255) & ~(size_t)255;
+            } else if (r->kind == 2) {
+                std::memcpy(w.h_in + b, r->field.data(), r->field.size() * sizeof(float));
+                b += ((r->field.size() * sizeof(float)) + 255) & ~(size_t)255;
+            }
+        }
+        OPH_CHECK(hipMemcpyAsync(w.d_in, w.h_in, in_bytes, hipMemcpyHostToDevice, s));
+        size_t ro = 0;
+        b = 0;
+        for (Request *r : batch) {
+            int rc = OPENPH_OK;
+            if (r->kind == 1) {
+                rc = build_vr(c, reinterpret_cast<const double *>(w.d_in + b), r->n, r->pdim,
+                              c.h_rows + ro, c.h_cols + ro, nullptr, nullptr, s);
+                b += ((r->points.size() * sizeof(double)) + 255) & ~(size_t)255;
+            } else if (r->kind == 2) {
+                rc = build_cubical(c, reinterpret_cast<const float *>(w.d_in + b), r->s0, r->s1,
+                                   r->s2, c.h_rows + ro, c.h_cols + ro, nullptr, nullptr, s);
+                b += ((r->field.size() * sizeof(float)) + 255) & ~(size_t)255;
+            }
+            if (rc) {
+                (void)hipStreamSynchronize(s);
+                return rc;
+            }
+            ro += r->nnz;
+        }
     }
     RunCfg cfg = cfg_from(&srv->opt.reduce, (int)N);
     openph_stats st;
@@ -226,6 +277,10 @@ static void worker_main(openph_server *srv, Worker *w) {
                 r->rows.shrink_to_fit();
                 r->cols.clear();
                 r->cols.shrink_to_fit();
+                r->points.clear();
+                r->points.shrink_to_fit();
+                r->field.clear();
+                r->field.shrink_to_fit();
                 srv->columns_done += r->m;
             }
             srv->batches++;
@@ -273,6 +328,31 @@ int openph_server_create(const openph_server_options *opt, openph_server **out) 
     return OPENPH_OK;
 }
 
+}  // extern "C"
+
+namespace oph {
+
+// Queues a request; its footprint decides which batch it may join.
+static int enqueue(openph_server *srv, std::unique_ptr<Request> r, int64_t *ticket) {
+    const int slot0 = srv->opt.reduce.slot_elems > 0 ? srv->opt.reduce.slot_elems : 32;
+    r->footprint = footprint(r->m, r->nnz, slot0);
+    {
+        std::lock_guard<std::mutex> g(srv->mu);
+        if ((int)srv->queue.size() >= srv->opt.max_queue) return OPENPH_EBUSY;
+        r->id = srv->next_id++;
+        r->t_submit = now_s();
+        *ticket = r->id;
+        srv->queue.push_back(r.get());
+        srv->requests[r->id] = std::move(r);
+    }
+    srv->cv_work.notify_one();
+    return OPENPH_OK;
+}
+
+}  // namespace oph
+
+extern "C" {
+
 int openph_server_submit(openph_server *srv, const int *rows1, const int *cols1, int m, int nnz,
                          int64_t *ticket) {
     if (!srv || !ticket || m < 0 || nnz < 0 || (nnz > 0 && (!rows1 || !cols1)))
@@ -292,19 +372,47 @@ int openph_server_submit(openph_server *srv, const int *rows1, const int *cols1,
     r->nnz = nnz;
     r->rows.assign(rows1, rows1 + nnz);
     r->cols.assign(cols1, cols1 + nnz);
-    const int slot0 = srv->opt.reduce.slot_elems > 0 ? srv->opt.reduce.slot_elems : 32;
-    r->footprint = footprint(m, nnz, slot0);
-    {
-        std::lock_guard<std::mutex> g(srv->mu);
-        if ((int)srv->queue.size() >= srv->opt.max_queue) return OPENPH_EBUSY;
-        r->id = srv->next_id++;
-        r->t_submit = now_s();
-        *ticket = r->id;
-        srv->queue.push_back(r.get());
-        srv->requests[r->id] = std::move(r);
-    }
-    srv->cv_work.notify_one();
-    return OPENPH_OK;
+    return enqueue(srv, std::move(r), ticket);
+}
+
+int openph_server_submit_vr(openph_server *srv, const double *points, int n, int dim,
+                            int64_t *ticket) {
+    if (!srv || !ticket || n < 0 || dim < 1 || dim > 7 || (n > 0 && !points)) return OPENPH_EINVAL;
+    int64_t m = 0, nnz = 0;
+    int rc = openph_vr2_size(n, &m, &nnz);
+    if (rc) return rc;
+    if (m > 0x7fffffffLL || nnz > 0x7fffffffLL) return OPENPH_EINVAL;
+    for (long long k = 0; k < (long long)n * dim; k++)
+        if (!std::isfinite(points[k])) return OPENPH_EINVAL;
+    auto r = std::make_unique<Request>();
+    r->kind = 1;
+    r->m = (int)m;
+    r->nnz = (int)nnz;
+    r->n = n;
+    r->pdim = dim;
+    r->points.assign(points, points + (size_t)n * dim);
+    return enqueue(srv, std::move(r), ticket);
+}
+
+int openph_server_submit_cubical(openph_server *srv, const float *field, int s0, int s1, int s2,
+                                 int64_t *ticket) {
+    if (!srv || !ticket || !field) return OPENPH_EINVAL;
+    int64_t m = 0, nnz = 0;
+    int rc = openph_cubical_size(s0, s1, s2, &m, &nnz);
+    if (rc) return rc;
+    if (m >= (1LL << 28) || nnz > 0x7fffffffLL) return OPENPH_EINVAL;
+    const size_t cells = (size_t)s0 * s1 * s2;
+    for (size_t k = 0; k < cells; k++)
+        if (std::isnan(field[k])) return OPENPH_EINVAL;
+    auto r = std::make_unique<Request>();
+    r->kind = 2;
+    r->m = (int)m;
+    r->nnz = (int)nnz;
+    r->s0 = s0;
+    r->s1 = s1;
+    r->s2 = s2;
+    r->field.assign(field, field + cells);
+    return enqueue(srv, std::move(r), ticket);
 }
 
 int openph_server_wait(openph_server *srv, int64_t ticket, int *low1, int *ess,

@@ -56,6 +56,13 @@ def _bind():
     L.openph_server_submit.restype = ctypes.c_int
     L.openph_server_submit.argtypes = [ctypes.c_void_p, ip, ip, ctypes.c_int, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_int64)]
+    L.openph_server_submit_vr.restype = ctypes.c_int
+    L.openph_server_submit_vr.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+    L.openph_server_submit_cubical.restype = ctypes.c_int
+    L.openph_server_submit_cubical.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_int,
+                                               ctypes.POINTER(ctypes.c_int64)]
     L.openph_server_wait.restype = ctypes.c_int
     L.openph_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_int64, ip, ip,
                                      ctypes.POINTER(RequestInfo)]
@@ -102,6 +109,32 @@ class Server:
     def submit(self, boundary) -> int:
         r, c = boundary.coo1()
         return self.submit_coo(r, c, boundary.m)
+
+    def submit_points(self, points) -> int:
+        """A VR 2-skeleton request: (n, d) points, d <= 7; the worker builds
+        the matrix on the device (openph_vr2_device's order)."""
+        p = np.ascontiguousarray(points, dtype=np.float64)
+        n, d = p.shape
+        t = ctypes.c_int64()
+        rc = _bind().openph_server_submit_vr(self._h, p.ctypes.data, n, d, ctypes.byref(t))
+        if rc != 0:
+            raise OpenPHError(rc, "openph_server_submit_vr")
+        self._m[t.value] = n + n * (n - 1) // 2 + n * (n - 1) * (n - 2) // 6
+        return t.value
+
+    def submit_field(self, field) -> int:
+        """A cubical lower-star request: an (s0, s1, s2) float32 grid; the
+        worker builds the matrix on the device (openph_cubical_device's
+        order)."""
+        f = np.ascontiguousarray(field, dtype=np.float32)
+        s0, s1, s2 = f.shape
+        t = ctypes.c_int64()
+        rc = _bind().openph_server_submit_cubical(self._h, f.ctypes.data, s0, s1, s2,
+                                                  ctypes.byref(t))
+        if rc != 0:
+            raise OpenPHError(rc, "openph_server_submit_cubical")
+        self._m[t.value] = (2 * s0 - 1) * (2 * s1 - 1) * (2 * s2 - 1)
+        return t.value
 
     def wait(self, ticket: int):
         """Returns (low0, ess, RequestInfo); low0 is 0-based (-1 = zero)."""

@@ -102,3 +102,36 @@ def test_server_rejects_malformed_request_without_poisoning_others():
         for t in (tg0, tg):
             low, ess, info = srv.wait(t)
             assert (low + 1).tolist() == [0, 0, 0, 2, 3, 0, 6]
+
+
+def test_server_builds_point_clouds_and_fields_on_the_device():
+    """Requests given as point clouds and grids, mixed with COO requests in
+    the same batches: each result equals the host-built matrix reduced
+    alone."""
+    from openph_amd.server import Server
+    print("server built requests", flush=True)
+    rng = np.random.default_rng(11)
+    items = []
+    for k in range(10):
+        if k % 3 == 2:
+            f = C.smoothed_field(rng, int(rng.integers(6, 14)))
+            items.append(("field", f, C.cubical_lower_star(f)))
+        else:
+            p = C.points_cube(rng, int(rng.integers(5, 30)))
+            items.append(("points" if k % 3 == 0 else "coo", p, C.vr_2skeleton(p)))
+    with Server(workers=2) as srv:
+        tickets = []
+        for kind, x, b in items:
+            if kind == "field":
+                tickets.append(srv.submit_field(x))
+            elif kind == "points":
+                tickets.append(srv.submit_points(x))
+            else:
+                tickets.append(srv.submit(b))
+        for t, (kind, x, b) in zip(tickets, items):
+            low, ess, info = srv.wait(t)
+            ref = oracle.pms(b)
+            assert np.array_equal(low, ref.low) and np.array_equal(ess, ref.ess), kind
+            assert info.m == b.m
+        with pytest.raises(openph_amd.OpenPHError):
+            srv.submit_points(np.full((4, 3), np.nan))
