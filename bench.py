@@ -246,7 +246,89 @@ def served_leg(workers: int = 4, seed: int = 0):
     out["vr_cubical"] = _served_stream(
         pool, [i % len(pool) for i in range(72)], workers, seed,
         "72 requests cycling over 24 (seed 0): 70% VR N~U{30..200}, 30% cubical side~U{16..96}")
+    out["vr_cubical_built"] = _served_built_stream(
+        [i % 24 for i in range(72)], workers, seed,
+        "the vr_cubical stream given as point clouds and fields, built on the device")
     return out
+
+
+def _served_built_stream(seq, workers, seed, what):
+    """Requests given as raw inputs (points / fields). Plain: each request
+    uploaded, built (openph_vr2_device / openph_cubical_device) and reduced
+    (openph_pms_device) alone, results copied back, one after another.
+    Served: the server builds each request on its worker's stream inside the
+    batch."""
+    import threading
+
+    import torch
+
+    import openph_amd
+    from openph_amd import complexes as C
+    from openph_amd.server import Server
+
+    rng = np.random.default_rng(seed)
+    inputs = [C.served_input(rng) for _ in range(24)]
+
+    def plain_one(kind, x):
+        t = torch.from_numpy(x).cuda()
+        rows, cols, vals, _ = (openph_amd.vr2_device(t) if kind == "points"
+                               else openph_amd.cubical_device(t))
+        m = vals.numel()
+        low = torch.empty(m, dtype=torch.int32, device="cuda")
+        ess = torch.empty_like(low)
+        openph_amd.pms_device(rows, cols, m, low, ess)
+        return low.cpu().numpy() - 1, ess.cpu().numpy()
+
+    ref = [plain_one(k, x) for k, x in inputs]  # warm-up and outputs alone
+    cols = sum(ref[i][0].shape[0] for i in seq)
+    t0 = time.perf_counter()
+    for i in seq:
+        plain_one(*inputs[i])
+    t_plain = time.perf_counter() - t0
+
+    def run(qps, rng2):
+        n_requests = len(seq)
+        arrivals = (np.zeros(n_requests) if qps is None
+                    else np.cumsum(rng2.exponential(1.0 / qps, n_requests)))
+        tickets = [None] * n_requests
+        with Server(workers=workers) as srv:
+            def submitter():
+                start = time.perf_counter()
+                for k, i in enumerate(seq):
+                    dt = arrivals[k] - (time.perf_counter() - start)
+                    if dt > 0:
+                        time.sleep(dt)
+                    kind, x = inputs[i]
+                    tickets[k] = srv.submit_points(x) if kind == "points" else srv.submit_field(x)
+
+            th = threading.Thread(target=submitter)
+            th.start()
+            lat, first, last, same = [], None, None, True
+            for k, i in enumerate(seq):
+                while tickets[k] is None:
+                    time.sleep(0.0002)
+                low, ess, info = srv.wait(tickets[k])
+                same &= bool(np.array_equal(low, ref[i][0]) and np.array_equal(ess, ref[i][1]))
+                lat.append(info.done_s - info.submit_s)
+                first = info.submit_s if first is None else min(first, info.submit_s)
+                last = info.done_s if last is None else max(last, info.done_s)
+            th.join()
+        lat_ms = np.array(lat) * 1e3
+        return {"qps": qps, "served_cols_per_s": cols / (last - first), "span_s": last - first,
+                "latency_ms_p50": float(np.percentile(lat_ms, 50)),
+                "latency_ms_p99": float(np.percentile(lat_ms, 99)), "served_equals_alone": same}
+
+    burst = run(None, np.random.default_rng(seed + 1))
+    paced = run(len(seq) / t_plain, np.random.default_rng(seed + 1))
+    return {
+        "workload": what, "requests": len(seq), "columns": cols,
+        "plain_build_and_reduce_cols_per_s": cols / t_plain, "plain_s": t_plain,
+        "served_cols_per_s": burst["served_cols_per_s"],
+        "speedup_served_vs_plain": burst["served_cols_per_s"] / (cols / t_plain),
+        "latency_ms_p50": paced["latency_ms_p50"], "latency_ms_p99": paced["latency_ms_p99"],
+        "burst": burst, "poisson": paced,
+        "served_equals_alone": burst["served_equals_alone"] and paced["served_equals_alone"],
+    }
 
 
 def _served_stream(pool, seq, workers, seed, what):

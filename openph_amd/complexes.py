@@ -37,6 +37,7 @@ __all__ = [
     "cubical64",
     "batched_mix",
     "served_request",
+    "served_input",
     "random_simplicial",
 ]
 
@@ -320,15 +321,26 @@ def batched_mix(count: int = 512, seed: int = 0, nmin: int = 20, nmax: int = 80)
     return out
 
 
-def served_request(rng: np.random.Generator, vr_range=(30, 200), side_range=(16, 96),
-                   p_vr: float = 0.7) -> Boundary:
-    """One request of the serving mix: 70% VR 2-skeleton of N ~ U{30..200}
-    uniform points, 30% cubical lower-star grid of side ~ U{16..96}."""
+def served_input(rng: np.random.Generator, vr_range=(30, 200), side_range=(16, 96),
+                 p_vr: float = 0.7):
+    """The raw input of one request of the serving mix: ("points", N x 3)
+    for a VR 2-skeleton of N ~ U{30..200} uniform points (70%), or ("field",
+    side^3 float32) for a cubical lower-star grid of side ~ U{16..96}."""
     if rng.uniform() < p_vr:
         n = int(rng.integers(vr_range[0], vr_range[1] + 1))
-        return vr_2skeleton(points_cube(rng, n), name=f"vr{n}")
+        return "points", points_cube(rng, n)
     side = int(rng.integers(side_range[0], side_range[1] + 1))
-    return cubical_lower_star(smoothed_field(rng, side), name=f"cubical{side}")
+    return "field", smoothed_field(rng, side)
+
+
+def served_request(rng: np.random.Generator, vr_range=(30, 200), side_range=(16, 96),
+                   p_vr: float = 0.7) -> Boundary:
+    """One request of the serving mix as a boundary matrix (served_input,
+    built on the host)."""
+    kind, x = served_input(rng, vr_range, side_range, p_vr)
+    if kind == "points":
+        return vr_2skeleton(x, name=f"vr{x.shape[0]}")
+    return cubical_lower_star(x, name=f"cubical{x.shape[0]}")
 
 
 def random_simplicial(rng: np.random.Generator, n_vertices: int, p_edge: float = 0.5,
