@@ -338,7 +338,7 @@ int orc_pms(int m, const int64_t *colptr, const int32_t *rows, int32_t *low,
     int rc = ORC_OK;
     dyn_store s;
     size_t mm = (size_t)m + 1;
-    int32_t *arglow = malloc(sizeof(int32_t) * mm), *classes = malloc(sizeof(int32_t) * mm);
+    int32_t *arglow = malloc(sizeof(int32_t) * mm), *classes = calloc(mm, sizeof(int32_t));
     int32_t *left = malloc(sizeof(int32_t) * mm), *beta = malloc(sizeof(int32_t) * mm);
     int32_t *pd = malloc(sizeof(int32_t) * mm), *visited = malloc(sizeof(int32_t) * mm);
     int32_t *clear = malloc(sizeof(int32_t) * mm);
