@@ -100,7 +100,9 @@ void launch_pool_cap(const Dev &d, unsigned long long requested, unsigned long l
 void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t s);
 void launch_slots(const Dev &d, hipStream_t s);
 void launch_phase0(const Dev &d, hipStream_t s);
-// Start of an iteration: iter + 1 and the per-iteration flags reset.
+// iter + 1 and the per-iteration flags reset, unless halted. The pms loop
+// does this in k_first; the serial algorithms use it to clear the overflow
+// flag before they resume.
 void launch_iter_begin(const Dev &d, hipStream_t s);
 // End of an iteration: progress, stall, overflow and pool-fill checks that
 // set Ctrl::halt.

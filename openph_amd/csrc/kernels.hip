@@ -442,13 +442,17 @@ void launch_apply1(const Dev &d, hipStream_t s) {
 // iteration returns at once while Ctrl::halt is set.
 // ---------------------------------------------------------------------------
 
-__global__ void k_iter_begin(Ctrl *c) {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || c->halt) return;
+__device__ __forceinline__ void iter_begin(Ctrl *c) {
     c->iter += 1;
     c->any_dup = 0;
     c->overflow = 0;
     c->nwork = 0;
     c->nlong = 0;
+}
+
+__global__ void k_iter_begin(Ctrl *c) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || c->halt) return;
+    iter_begin(c);
 }
 
 // After Phase II and its bookkeeping. While lows repeat, pms marks a
@@ -531,6 +535,9 @@ __global__ void __launch_bounds__(kBlock) k_first(Dev d) {
     __shared__ int hkey[kHash];
     __shared__ int hval[kHash];
     if (d.ctrl->halt) return;
+    // the iteration starts here: none of this kernel's blocks reads the
+    // flags reset below
+    if (blockIdx.x == 0 && threadIdx.x == 0) iter_begin(d.ctrl);
     for (int i = threadIdx.x; i < kHash; i += kBlock) {
         hkey[i] = -1;
         hval[i] = kIntMax;
@@ -1373,27 +1380,29 @@ void launch_serial(const Dev &d, const int *order, int n, bool twist, unsigned l
 // After Phase II: publish new pivots, clear_positives, ess_hat
 // (kernels.cu:113-124) and update_classes (kernels.cu:93-101). Every thread
 // touches its own column first, so one pass suffices.
-__global__ void k_apply2(Dev d) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= d.m || d.ctrl->halt) return;
-    int np = d.newpiv[t];
-    if (np > -1) {
-        publish_pivot(d, t, np);
-        d.newpiv[t] = -1;
-    }
-    if (d.clear[t]) {
-        d.low[t] = -1;
-        d.classes[t] = 1;
-        d.clear[t] = 0;
-    }
-    int lt = d.low[t];
-    if (lt > -1) d.ess[lt] = 0;
-    int a = d.arglow[t];
-    if (a > -1) {
-        d.ess[a] = 0;
-        d.ess[t] = 0;
-        d.classes[a] = -1;
-        d.classes[t] = 1;
+__global__ void __launch_bounds__(kBlock) k_apply2(Dev d) {
+    if (d.ctrl->halt) return;  // uniform over the grid
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < d.m) {
+        int np = d.newpiv[t];
+        if (np > -1) {
+            publish_pivot(d, t, np);
+            d.newpiv[t] = -1;
+        }
+        if (d.clear[t]) {
+            d.low[t] = -1;
+            d.classes[t] = 1;
+            d.clear[t] = 0;
+        }
+        int lt = d.low[t];
+        if (lt > -1) d.ess[lt] = 0;
+        int a = d.arglow[t];
+        if (a > -1) {
+            d.ess[a] = 0;
+            d.ess[t] = 0;
+            d.classes[a] = -1;
+            d.classes[t] = 1;
+        }
     }
 }
 

@@ -183,8 +183,7 @@ struct Shape {
 // stamps, when either is given.
 static int enqueue_iteration(const Dev &d, const Shape &sh, hipStream_t s, hipEvent_t e0,
                              hipEvent_t e1, unsigned long long *stamps) {
-    launch_iter_begin(d, s);
-    launch_phase1_detect(d, s);
+    launch_phase1_detect(d, s);  // k_first starts the iteration
     launch_phase1_mark(d, s);
     launch_apply1(d, s);
     if (e0) OPH_CHECK(hipEventRecord(e0, s));
@@ -426,8 +425,7 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
                 if ((rc = grow_events(evs, 2 * (size_t)next + 2))) return rc;
                 OPH_CHECK(hipEventRecord(evs[2 * next], s));
             }
-            launch_iter_begin(d, s);
-            launch_phase1_detect(d, s);
+            launch_phase1_detect(d, s);  // k_first starts the iteration
             launch_phase1_mark(d, s);
             launch_apply1(d, s);
             if ((rc = read_ctrl(c, s))) return rc;
