@@ -378,3 +378,21 @@ def random_simplicial(rng: np.random.Generator, n_vertices: int, p_edge: float =
     dim = np.array([len(s) - 1 for s in order], dtype=np.int32)
     value = np.array([val[s] for s in order])
     return Boundary(colptr=colptr, rows=rows, dim=dim, value=value, name="random")
+
+
+def random_upper(rng: np.random.Generator, m: int, kmax: int = 5) -> Boundary:
+    """A random strictly upper-triangular Z/2 matrix with 0..kmax entries per
+    column. It is not a boundary matrix (its square is not zero), so only the
+    algorithms that do not clear columns apply to it: standard and the row
+    algorithms. Columns with one low then often have different lengths, which
+    tells ph_row (neighbours of its pivot's pseudo-dimension only) from
+    ph_row_parallel."""
+    counts = np.array([rng.integers(0, min(j, kmax) + 1) for j in range(m)], dtype=np.int64)
+    colptr = _csc_from_lists(counts, None)
+    rows = np.empty(int(colptr[-1]), dtype=np.int32)
+    for j in range(m):
+        if counts[j]:
+            rows[colptr[j]:colptr[j + 1]] = np.sort(rng.choice(j, size=int(counts[j]),
+                                                               replace=False))
+    return Boundary(colptr=colptr, rows=rows, dim=np.maximum(counts - 1, 0).astype(np.int32),
+                    value=np.arange(m, dtype=np.float64), name=f"upper{m}")

@@ -134,7 +134,23 @@ void launch_apply2(const Dev &d, hipStream_t s);
 // after column k, *start (when given) before the first.
 void launch_serial(const Dev &d, const int *order, int n, bool twist, unsigned long long *stamps,
                    unsigned long long *start, hipStream_t s);
+// The row algorithms (ph_row, and ph_row_parallel with all_dims): lists of
+// the columns of each low, then one workgroup over the rows from state[0]
+// down. stamps[m-1-i] (may be null) gets the device clock after row i, start
+// (may be null) the clock before the first row. A full pool stops the kernel
+// with Ctrl::overflow = 1 and the resume point in state; Ctrl::nwork counts
+// the additions and rows done by the launch.
+struct RowLists {
+    int *head, *next, *pivot, *plen, *cand, *state;
+};
+void launch_row_lists(const Dev &d, const RowLists &r, hipStream_t s);
+void launch_ph_row(const Dev &d, const RowLists &r, bool all_dims, unsigned long long *stamps,
+                   unsigned long long *start, hipStream_t s);
+// ess = 1 except at the lows (ph_row_parallel's estimate).
+void launch_ess_at_lows(const Dev &d, hipStream_t s);
 void launch_final(const Dev &d, int *low1_out, int *ess_out, hipStream_t s);
+// 1-based low and ess out, with no final ess pass.
+void launch_output(const Dev &d, int *low1_out, int *ess_out, hipStream_t s);
 // ess_hat_final (kernels.cu:103-111) alone, and update_classes_final
 // (kernels.cu:79-91) on classes[]: the state the reference tracks after its
 // last iteration. Needs k_final_ess first; classes[] is not used afterwards.

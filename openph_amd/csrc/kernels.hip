@@ -1377,6 +1377,233 @@ void launch_serial(const Dev &d, const int *order, int n, bool twist, unsigned l
     k_serial<<<1, kLongBlock, 0, s>>>(d, order, n, twist ? 1 : 0, stamps, start);
 }
 
+// ---------------------------------------------------------------------------
+// The row algorithm (ph_row.cu:38-121, ph_row_parallel.cu:20-84) on the
+// device. Rows are taken from the last to the first; the leftmost column
+// whose low is the row (its pivot) is added to every other column with that
+// low, which moves to a smaller low. ph_row takes only the columns of the
+// pivot's pseudo-dimension (get_neighbours_position_host walks the pivot's
+// dimension list, ph_row.cu:25-36); ph_row_parallel takes every column with
+// that low (mark_neighbours, ph_row_parallel.cu:2-10).
+//
+// One workgroup of kLongBlock lanes takes the rows in turn. The columns of
+// each low are kept in linked lists (head[row], next[column]) so that a row
+// finds them without a pass over all columns; a list may still hold a column
+// whose low has moved on, which is skipped. For each row:
+//   1. lane 0 walks the row's list and gathers its neighbours into cand[];
+//   2. the block adds the pivot (staged in LDS once) to each neighbour;
+//   3. the lanes push every moved neighbour onto the list of its new low and
+//      mark ess.
+// classes are not kept: they feed only the trackers, which the row
+// algorithms leave as the caller set them, and the pool compaction drops the
+// columns marked positive, which here may still be reduced.
+// Within a row the additions are independent of each other, so their order
+// does not matter but for one thing: the host loop marks a neighbour when
+// it is, at that moment, the leftmost column of its new low
+// (left_to_right_neighbours_host, ph_row.cu:14-21), and in its ascending
+// order that is exactly the neighbour that is leftmost once the whole row is
+// done. Step 3 marks that one, after the row.
+// pivot[row] is the leftmost column that has had that low; a column's low
+// only decreases, so it enters each list at most once and cand[] never holds
+// a column twice.
+// A full pool stops the kernel inside step 2; state = {row, next candidate,
+// candidates} lets the host compact and start it again where it stopped.
+// ---------------------------------------------------------------------------
+
+// Lists, leftmost columns and pseudo-dimensions (initial lengths) of the
+// initial lows; the resume state points at the last row.
+__global__ void k_row_lists(Dev d, RowLists r) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j == 0) {
+        r.state[0] = d.m - 1;
+        r.state[1] = 0;
+        r.state[2] = -1;
+    }
+    if (j >= d.m) return;
+    r.plen[j] = d.len[j];
+    const int l = d.low[j];
+    if (l < 0) return;
+    r.next[j] = atomicExch(&r.head[l], j);
+    atomicMin(&r.pivot[l], j);
+}
+
+__global__ void k_fill_i(int *p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void __launch_bounds__(kLongBlock) k_ph_row(Dev d, RowLists r, int all_dims,
+                                                      unsigned long long *stamps,
+                                                      unsigned long long *start) {
+    constexpr int W = kLongW;
+    __shared__ int lds[3 * W];
+    __shared__ int s_scan[kLongBlock / 64];
+    __shared__ unsigned long long s_word;
+    __shared__ int s_n;
+    const BlockTeam<kLongBlock> t{s_scan, &s_word};
+    const int gl = threadIdx.x;
+    int *sP = lds, *sB = lds + W, *sC = lds + 2 * W;
+    Phase2Counters ctr;
+    if (start && gl == 0) *start = (unsigned long long)wall_clock64();
+    int i = r.state[0], q = r.state[1], n = r.state[2];
+    int progress = 0;
+    bool stop = false;
+    for (; i >= 0; i--, q = 0, n = -1) {
+        const int pv = r.pivot[i];
+        if (pv != kIntMax) {
+            if (n < 0) {  // step 1
+                if (gl == 0) {
+                    // a list holds each column at most once: a longer walk
+                    // is a broken list, reported instead of followed
+                    const int pl = r.plen[pv];
+                    int c = 0, steps = 0;
+                    for (int j = __hip_atomic_load(&r.head[i], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                         j >= 0; j = r.next[j]) {
+                        if (++steps > d.m || j >= d.m) {
+                            atomicOr(&d.ctrl->err, kErrState);
+                            break;
+                        }
+                        if (j != pv && d.low[j] == i && (all_dims || r.plen[j] == pl))
+                            r.cand[c++] = j;
+                    }
+                    s_n = c;
+                }
+                t.fence();
+                n = s_n;
+                q = 0;
+            }
+            const unsigned poff = d.off[pv];
+            const int np = d.len[pv];
+            const bool pin = np <= W;
+            if (n > q && pin)
+                for (int k = gl; k < np; k += kLongBlock) sP[k] = d.pool[poff + k];
+            t.sync();
+            for (; q < n; q++) {  // step 2
+                const int j = r.cand[q];
+                const unsigned coff = d.off[j];
+                const int clen = d.len[j], ccap = d.cap[j];
+                const bool cin = clen <= W;
+                if (cin)
+                    for (int k = gl; k < clen; k += kLongBlock) sB[k] = d.pool[coff + k];
+                t.sync();
+                int nc, nl;
+                unsigned roff = coff;
+                int rcap = ccap;
+                if (pin && cin && np + clen <= W) {
+                    nc = team_xor(t, sP, np, sB, clen, sC);
+                    t.sync();
+                    if (nc > rcap) {
+                        unsigned want = (unsigned)nc;
+                        if (want < (unsigned)d.slot0) want = (unsigned)d.slot0;
+                        want = (want + 7u) & ~7u;
+                        const unsigned long long base = team_alloc(t, d, want);
+                        if (base == ~0ull) {
+                            stop = true;
+                            break;
+                        }
+                        roff = (unsigned)base;
+                        rcap = (int)want;
+                    }
+                    for (int k = gl; k < nc; k += kLongBlock) d.pool[roff + k] = sC[k];
+                    nl = nc ? sC[nc - 1] : -1;
+                } else {
+                    const unsigned long long base = team_alloc(t, d, (unsigned)(np + clen));
+                    if (base == ~0ull) {
+                        stop = true;
+                        break;
+                    }
+                    nc = team_xor(t, pin ? sP : d.pool + poff, np, cin ? sB : d.pool + coff,
+                                  clen, d.pool + base);
+                    t.fence();
+                    roff = (unsigned)base;
+                    rcap = np + clen;
+                    ctr.spills += cin ? 1 : 0;
+                    nl = nc ? d.pool[base + nc - 1] : -1;
+                }
+                ctr.nadd++;
+                ctr.nel += (unsigned long long)(np + clen + nc);
+                progress++;
+                if (gl == 0) {
+                    if (nl >= i) atomicOr(&d.ctrl->err, kErrState);
+                    d.off[j] = roff;
+                    d.len[j] = nc;
+                    d.cap[j] = rcap;
+                    d.low[j] = nl;
+                    if (nl > -1 && j < r.pivot[nl]) r.pivot[nl] = j;
+                }
+                t.fence();  // sB and sC are reused by the next neighbour
+            }
+            if (stop) break;
+            for (int k = gl; k < n; k += kLongBlock) {  // step 3
+                const int j = r.cand[k];
+                const int nl = d.low[j];
+                if (nl < 0) continue;
+                r.next[j] = atomicExch(&r.head[nl], j);
+                if (!all_dims && r.pivot[nl] == j) {
+                    d.ess[j] = 0;
+                    d.ess[nl] = 0;
+                }
+            }
+            if (gl == 0 && n > 0 && !all_dims) {  // ph_row.cu:6-9
+                d.ess[i] = 0;
+                d.ess[pv] = 0;
+            }
+            t.fence();
+        }
+        progress++;
+        if (stamps && gl == 0) stamps[d.m - 1 - i] = (unsigned long long)wall_clock64();
+    }
+    if (gl == 0) {
+        if (stop) {
+            r.state[0] = i;
+            r.state[1] = q;
+            r.state[2] = n;
+            d.ctrl->overflow = 1;
+        } else {
+            r.state[0] = -1;
+            r.state[1] = 0;
+            r.state[2] = -1;
+        }
+        d.ctrl->nwork = progress;
+        if (ctr.nadd) {
+            atomicAdd(&d.ctrl->additions, ctr.nadd);
+            atomicAdd(&d.ctrl->merge_elems, ctr.nel);
+        }
+        if (ctr.spills) atomicAdd(&d.ctrl->spills, ctr.spills);
+    }
+}
+
+// ph_row_parallel's estimate (ess_hat, kernels.cu:113-124, with arglow never
+// set by the row algorithm): 0 exactly at the rows that are some column's
+// low after some row, i.e. at the final lows, and 1 everywhere else.
+__global__ void k_ess_lows_mark(Dev d) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.m) return;
+    const int lt = d.low[t];
+    if (lt > -1) d.ess[lt] = 0;
+}
+
+void launch_row_lists(const Dev &d, const RowLists &r, hipStream_t s) {
+    const int gm = (d.m + kBlock - 1) / kBlock;
+    if (!d.m) return;
+    k_fill_i<<<gm, kBlock, 0, s>>>(r.head, d.m, -1);
+    k_fill_i<<<gm, kBlock, 0, s>>>(r.pivot, d.m, kIntMax);
+    k_row_lists<<<gm, kBlock, 0, s>>>(d, r);
+}
+
+void launch_ph_row(const Dev &d, const RowLists &r, bool all_dims, unsigned long long *stamps,
+                   unsigned long long *start, hipStream_t s) {
+    k_ph_row<<<1, kLongBlock, 0, s>>>(d, r, all_dims ? 1 : 0, stamps, start);
+}
+
+void launch_ess_at_lows(const Dev &d, hipStream_t s) {
+    const int gm = (d.m + kBlock - 1) / kBlock;
+    if (!d.m) return;
+    k_fill_i<<<gm, kBlock, 0, s>>>(d.ess, d.m, 1);
+    k_ess_lows_mark<<<gm, kBlock, 0, s>>>(d);
+}
+
 // After Phase II: publish new pivots, clear_positives, ess_hat
 // (kernels.cu:113-124) and update_classes (kernels.cu:93-101). Every thread
 // touches its own column first, so one pass suffices.
@@ -1438,6 +1665,10 @@ void launch_final(const Dev &d, int *low1_out, int *ess_out, hipStream_t s) {
     if (!d.m) return;
     k_final_ess<<<gm, kBlock, 0, s>>>(d);
     k_output<<<gm, kBlock, 0, s>>>(d, low1_out, ess_out);
+}
+
+void launch_output(const Dev &d, int *low1_out, int *ess_out, hipStream_t s) {
+    if (d.m) k_output<<<(d.m + kBlock - 1) / kBlock, kBlock, 0, s>>>(d, low1_out, ess_out);
 }
 
 void launch_final_ess(const Dev &d, hipStream_t s) {

@@ -590,18 +590,14 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
     return OPENPH_OK;
 }
 
-// standard and twist (and their _parallel names) run their own column
-// order on the device; the other algorithm strings run pms.
-static bool serial_algo(const char *a, bool *twist) {
-    if (!std::strcmp(a, "standard") || !std::strcmp(a, "standard_parallel")) {
-        *twist = false;
-        return true;
-    }
-    if (!std::strcmp(a, "twist") || !std::strcmp(a, "twist_parallel")) {
-        *twist = true;
-        return true;
-    }
-    return false;
+// The algorithms other than pms run their own order on the device.
+enum SerialAlgo { kAlgoPms = 0, kAlgoStandard, kAlgoTwist, kAlgoRow, kAlgoRowAll };
+static SerialAlgo serial_algo(const char *a) {
+    if (!std::strcmp(a, "standard") || !std::strcmp(a, "standard_parallel")) return kAlgoStandard;
+    if (!std::strcmp(a, "twist") || !std::strcmp(a, "twist_parallel")) return kAlgoTwist;
+    if (!std::strcmp(a, "ph_row")) return kAlgoRow;
+    if (!std::strcmp(a, "ph_row_parallel")) return kAlgoRowAll;
+    return kAlgoPms;
 }
 
 // standard (standard.cu:18-54) and twist (twist.cu:16-87) on the device.
@@ -609,12 +605,17 @@ static bool serial_algo(const char *a, bool *twist) {
 // - 1, dimensions.cu:54-104): standard takes every column in index order;
 // twist takes the pseudo-dimensions from the highest down to 1, each in index
 // order. num_iters is the reference's: one iteration per column taken, plus
-// one. time_track[i] is the device time of the i-th column; the err_*
-// trackers, which the reference evaluates over all m columns after every
-// column, are left as the caller set them.
+// one. time_track[i] is the device time of the i-th column.
+// ph_row (ph_row.cu:38-121) and ph_row_parallel (ph_row_parallel.cu:20-84)
+// take the rows from the last to the first: num_iters = m + 1 and
+// time_track[i] is the device time of the i-th row taken. ph_row_parallel's
+// ess is the reference's: 0 exactly at the lows.
+// For all four the err_* trackers, which the reference evaluates over all m
+// columns after every step, are left as the caller set them.
 static int run_serial(Context &c, const int *rows1, const int *cols1, int m, int nnz, int *low1,
-                      int *ess, hipStream_t s, const RunCfg &cfg, bool twist, openph_stats *stats,
-                      const Track *tr) {
+                      int *ess, hipStream_t s, const RunCfg &cfg, SerialAlgo algo,
+                      openph_stats *stats, const Track *tr) {
+    const bool twist = algo == kAlgoTwist, rowalg = algo == kAlgoRow || algo == kAlgoRowAll;
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (m == 0) {
         if (stats) stats->num_iters = 1;
@@ -650,8 +651,9 @@ static int run_serial(Context &c, const int *rows1, const int *cols1, int m, int
     if ((rc = err_from_bits(c.h_ctrl->err))) return rc;
     lcap = c.h_ctrl->pool_cap;
     std::vector<int> order;
-    order.reserve((size_t)m);
-    if (!twist) {
+    if (rowalg) {
+    } else if (!twist) {
+        order.reserve((size_t)m);
         for (int j = 0; j < m; j++) order.push_back(j);
     } else {
         const int maxlen = c.h_ctrl->maxlen;
@@ -666,29 +668,45 @@ static int run_serial(Context &c, const int *rows1, const int *cols1, int m, int
         for (int j = 0; j < m; j++)
             if (len[j] >= 2) order[at[len[j]]++] = j;
     }
-    const int n = (int)order.size();
-    OPH_CHECK(grow(&c.s_order, c.c_sorder, (size_t)n + 1));
-    if (n) OPH_CHECK(hipMemcpyAsync(c.s_order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    // steps: columns taken, or rows for the row algorithms
+    const int n = rowalg ? m : (int)order.size();
+    OPH_CHECK(grow(&c.s_order, c.c_sorder, rowalg ? (size_t)5 * m + 4 : (size_t)n + 1));
+    if (n && !rowalg)
+        OPH_CHECK(hipMemcpyAsync(c.s_order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    const size_t mm = (size_t)m;
+    const RowLists rl{c.s_order,          c.s_order + mm,     c.s_order + 2 * mm,
+                      c.s_order + 3 * mm, c.s_order + 4 * mm, c.s_order + 5 * mm};
     const bool times = tr && tr->time_track;
     if (times) OPH_CHECK(grow(&c.s_stamps, c.c_sstamps, (size_t)n + 1));
     OPH_CHECK(hipEventRecord(c.ev0, s));
+    if (rowalg) launch_row_lists(d, rl, s);
     int k0 = 0, ncompact = 0, stalls = 0, launches = 0;
     const Ctrl &h = *c.h_ctrl;
     while (k0 < n) {
-        launch_serial(d, c.s_order + k0, n - k0, twist, times ? c.s_stamps + 1 + k0 : nullptr,
-                      times && k0 == 0 ? c.s_stamps : nullptr, s);
+        if (rowalg)
+            launch_ph_row(d, rl, algo == kAlgoRowAll, times ? c.s_stamps + 1 : nullptr,
+                          times && launches == 0 ? c.s_stamps : nullptr, s);
+        else
+            launch_serial(d, c.s_order + k0, n - k0, twist, times ? c.s_stamps + 1 + k0 : nullptr,
+                          times && k0 == 0 ? c.s_stamps : nullptr, s);
         launches++;
         if ((rc = read_ctrl(c, s))) return rc;
         if (h.err) return err_from_bits(h.err) ? err_from_bits(h.err) : OPENPH_EINVAL;
         if (!h.overflow) break;
-        // a full pool: compact, then take the same column up again
+        // a full pool: compact, then take the same column (or row) up again;
+        // nwork is the position reached, or the row kernel's steps done
         stalls = h.nwork == 0 ? stalls + 1 : 0;
         if (stalls >= 2) return OPENPH_ENOMEM;
-        k0 += h.nwork;
+        if (!rowalg) k0 += h.nwork;
         if ((rc = compact(c, cfg, s, &ncompact, lcap))) return rc;
         launch_iter_begin(d, s);  // clears the overflow flag
     }
-    launch_final(d, low1, ess, s);
+    if (algo == kAlgoRowAll) {
+        launch_ess_at_lows(d, s);
+        launch_output(d, low1, ess, s);
+    } else {
+        launch_final(d, low1, ess, s);
+    }
     OPH_CHECK(hipEventRecord(c.ev1, s));
     if ((rc = read_ctrl(c, s))) return rc;
     if (stats) {
@@ -759,9 +777,9 @@ static int run_host(const char *algstr, const int *rows1, const int *cols1, int 
         OPH_CHECK(hipMemcpyAsync(c->h_cols, cols1, sizeof(int) * nnz, hipMemcpyHostToDevice, s));
     }
     RunCfg cfg = cfg_from(opt, nnz);
-    bool twist = false;
-    if (serial_algo(algstr, &twist))
-        rc = run_serial(*c, c->h_rows, c->h_cols, m, nnz, c->h_low, c->h_ess, s, cfg, twist, stats,
+    const SerialAlgo algo = serial_algo(algstr);
+    if (algo != kAlgoPms)
+        rc = run_serial(*c, c->h_rows, c->h_cols, m, nnz, c->h_low, c->h_ess, s, cfg, algo, stats,
                         tr);
     else
         rc = run_pms(*c, c->h_rows, c->h_cols, m, nnz, c->h_low, c->h_ess, s, cfg, stats, tr);

@@ -71,7 +71,7 @@ def _lib():
                                 ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_void_p, ctypes.c_void_p]
         lib.orc_ph_row.restype = ctypes.c_int
-        lib.orc_ph_row.argtypes = [ctypes.c_int, i64p, i32p, i32p, i32p, i64p]
+        lib.orc_ph_row.argtypes = [ctypes.c_int, i64p, i32p, i32p, i32p, i64p, ctypes.c_int]
         _LIB = lib
     return _LIB
 
@@ -163,13 +163,15 @@ def pms(b, max_iters: int = 0, iter_stats: bool = False, snapshot: bool = False,
     return res
 
 
-def ph_row(b) -> OracleResult:
+def ph_row(b, parallel: bool = False) -> OracleResult:
+    """ph_row, or ph_row_parallel with ``parallel`` (every later column with
+    the pivot's low is a neighbour; ess from ess_hat after every row)."""
     colptr, rows = _csc(b)
     m = b.m
     low = np.empty(m, np.int32)
     ess = np.empty(m, np.int32)
     st = np.zeros(4, np.int64)
-    _check(_lib().orc_ph_row(m, colptr, rows, low, ess, st), "ph_row")
+    _check(_lib().orc_ph_row(m, colptr, rows, low, ess, st, 1 if parallel else 0), "ph_row")
     return OracleResult(low, ess, st)
 
 

@@ -487,8 +487,14 @@ done:
 /* Row algorithm (ph_row.cu:1-121).                                         */
 /* ---------------------------------------------------------------------- */
 
+/* all_dims = 0: ph_row (ph_row.cu:38-121), neighbours from the pivot's
+ * pseudo-dimension list. all_dims = 1: ph_row_parallel
+ * (ph_row_parallel.cu:20-84): every later column with the pivot's low
+ * (mark_neighbours, ph_row_parallel.cu:2-10), and ess from ess_hat
+ * (kernels.cu:113-124) after every row; arglow is never set by this
+ * algorithm, so only the lows are marked, and there is no final pass. */
 int orc_ph_row(int m, const int64_t *colptr, const int32_t *rows, int32_t *low,
-               int32_t *ess, int64_t *stats) {
+               int32_t *ess, int64_t *stats, int all_dims) {
     if (m < 0) return ORC_EINPUT;
     int rc = ORC_OK;
     dyn_store s;
@@ -516,25 +522,41 @@ int orc_ph_row(int m, const int64_t *colptr, const int32_t *rows, int32_t *low,
     int64_t st[ST_N] = {0, 0, 0, 0};
     for (int i = m - 1; i > -1 && rc == ORC_OK; i--) {
         int pv = pivots[i];
-        if (pv == -1) continue;
+        if (pv == -1) {
+            if (all_dims) /* ess_hat runs after every row */
+                for (int j = 0; j < m; j++)
+                    if (low[j] > -1) ess[low[j]] = 0;
+            continue;
+        }
         /* neighbours: later columns of the pivot's pseudo-dimension whose
          * low equals low[pv] (get_neighbours_position_host, ph_row.cu:27-37) */
         int cnt = 0;
-        for (int c = next[pv]; c != -1; c = next[c])
-            if (low[c] == low[pv]) nb[cnt++] = c;
+        if (all_dims) {
+            for (int c = pv + 1; c < m; c++)
+                if (low[c] == i) nb[cnt++] = c;
+        } else {
+            for (int c = next[pv]; c != -1; c = next[c])
+                if (low[c] == low[pv]) nb[cnt++] = c;
+        }
         for (int l = 0; l < cnt && rc == ORC_OK; l++) {
             int j = nb[l];
-            if (low[j] > -1) ess[low[j]] = 0;
-            ess[pv] = 0;
-            ess[low[pv]] = 0;
+            if (!all_dims) {
+                if (low[j] > -1) ess[low[j]] = 0;
+                ess[pv] = 0;
+                ess[low[pv]] = 0;
+            }
             int lj = dyn_add(&s, pv, j, st, &rc);
             low[j] = lj;
             if (lj != -1 && (pivots[lj] == -1 || j < pivots[lj])) pivots[lj] = j;
-            if (lj != -1 && pivots[lj] == j) { ess[j] = 0; ess[lj] = 0; }
+            if (!all_dims && lj != -1 && pivots[lj] == j) { ess[j] = 0; ess[lj] = 0; }
         }
+        if (all_dims)
+            for (int j = 0; j < m; j++)
+                if (low[j] > -1) ess[low[j]] = 0;
     }
-    for (int j = 0; j < m; j++)
-        if (low[j] > -1) { ess[j] = 0; ess[low[j]] = 0; }
+    if (!all_dims)
+        for (int j = 0; j < m; j++)
+            if (low[j] > -1) { ess[j] = 0; ess[low[j]] = 0; }
     st[ST_ITERS] = m;
     if (stats) memcpy(stats, st, sizeof(st));
 done:

@@ -259,3 +259,48 @@ def test_oracle_trackers(name, oracle_lib):
     lt2[j[0]] = -1
     bad = oracle.pms(b, snapshot=True, low_true=lt2).trk
     assert bad[-1][0] == lt[j[0]] + 1 and bad[-1][1] == lt[j[0]] + 1
+
+
+def _row_terminated(b, low):
+    """ph_row's end state: for every row, no column of its leftmost column's
+    pseudo-dimension (initial length) shares that low."""
+    plen = np.diff(b.colptr)
+    first = {}
+    for j in np.flatnonzero(low >= 0):
+        first.setdefault(int(low[j]), int(j))
+    for j in np.flatnonzero(low >= 0):
+        p = first[int(low[j])]
+        if j != p and plen[j] == plen[p]:
+            return False
+    return True
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_row_algorithms_on_random_matrices(seed, oracle_lib):
+    """ph_row_parallel (ph_row_parallel.cu:20-84) reduces every column, so
+    its low is the dense reduction's, and its ess (ess_hat with no arglow) is
+    0 exactly at the lows. ph_row (ph_row.cu:38-121) only reduces inside the
+    pivot's pseudo-dimension: on a matrix whose equal-low columns differ in
+    length it stops short of the reduction, in the state _row_terminated
+    describes, with lows never above the initial ones and ess <= the exact
+    estimate."""
+    b = C.random_upper(np.random.default_rng(2000 + seed), 200)
+    dl, de = oracle.dense_reduce(b)
+    par = oracle.ph_row(b, parallel=True)
+    assert np.array_equal(par.low, dl)
+    at_low = np.zeros(b.m, bool)
+    at_low[dl[dl >= 0]] = True
+    assert np.array_equal(par.ess, (~at_low).astype(np.int32))
+    row = oracle.ph_row(b)
+    assert not np.array_equal(row.low, dl)  # this matrix tells the two apart
+    assert _row_terminated(b, row.low)
+    counts = np.diff(b.colptr)
+    low0 = np.where(counts > 0, b.rows[np.maximum(b.colptr[1:] - 1, 0)], -1)
+    assert (row.low <= low0).all()
+    assert not _row_terminated(b, low0)
+    # the same additions on a complex, where equal lows mean equal lengths
+    c = C.random_simplicial(np.random.default_rng(seed), 7, 0.8, 0.7, 0.6)
+    cl, ce = oracle.dense_reduce(c)
+    r = oracle.ph_row(c)
+    assert np.array_equal(r.low, cl) and (r.ess <= ce).all()
+    assert r.additions == oracle.ph_row(c, parallel=True).additions
