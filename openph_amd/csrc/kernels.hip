@@ -248,11 +248,18 @@ __device__ __forceinline__ int lower_bound_i(const int *a, int n, int key) {
 // One thread per column: column extent by binary search in the sorted cols,
 // initial state (openph.cu:86-90 fills, kernels.cu:40-52 compute_low). The
 // COO start of the column goes to off2 until the pool layout is known.
+// A wave covers 63 columns: lane l searches the start of column 63w + l, and
+// lane 63 that of the next wave's first column, so every column's end comes
+// from its right neighbour lane and each lane searches once.
 __global__ void k_init_cols(Dev d, const int *__restrict__ rows1, const int *__restrict__ cols1) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= d.m) return;
-    int lo = lower_bound_i(cols1, d.nnz, j + 1);
-    int hi = lower_bound_i(cols1, d.nnz, j + 2);
+    const int lane = lane_id();
+    const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long jl = w * 63 + lane;
+    const int key = jl < d.m ? (int)jl + 1 : d.m + 1;
+    const int lo = lower_bound_i(cols1, d.nnz, key);
+    int hi = __shfl_down(lo, 1, 64);
+    if (lane == 63 || jl >= d.m) return;
+    const int j = (int)jl;
     if (hi < lo) hi = lo;  // unsorted input; flagged by k_entries
     int n = hi - lo;
     d.off2[j] = (unsigned)lo;
@@ -270,6 +277,11 @@ __global__ void k_init_cols(Dev d, const int *__restrict__ rows1, const int *__r
     d.newpiv[j] = -1;
     d.pivdesc[j] = make_int4(-1, 0, 0, 0);
     if (n) atomicMax(&d.ctrl->maxlen, n);
+}
+
+static int init_cols_blocks(int m) {
+    const long long waves = ((long long)m + 62) / 63;
+    return (int)((waves * 64 + kBlock - 1) / kBlock);
 }
 
 // Validation, pool copy (0-based) and left[] by atomicMin, kTile entries per
@@ -327,8 +339,8 @@ __global__ void k_beta(Dev d) {
 }
 
 void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t s) {
-    int gm = (d.m + kBlock - 1) / kBlock, ge = (d.nnz + kTile - 1) / kTile;
-    k_init_cols<<<gm, kBlock, 0, s>>>(d, rows1, cols1);
+    const int ge = (d.nnz + kTile - 1) / kTile;
+    if (d.m) k_init_cols<<<init_cols_blocks(d.m), kBlock, 0, s>>>(d, rows1, cols1);
     launch_scan(d, 0, d.off, s);
     if (d.nnz) k_entries<<<ge, kBlock, 0, s>>>(d, rows1, cols1);
 }
@@ -400,39 +412,59 @@ __global__ void k_phase0(Dev d) {
 // clear_positives (pms_functions.cu:131-140), reset of the first-occurrence
 // table and construction of the Phase II worklist: columns still unknown
 // whose low has a pivot to their left.
-__global__ void k_apply1(Dev d) {
+// One tile of kTile columns per block, kItems consecutive columns per
+// thread; the worklist gets the block's columns in column order with one
+// atomic per block (the appends of all blocks go to one counter, and
+// same-address atomics serialize).
+__global__ void __launch_bounds__(kBlock) k_apply1(Dev d) {
+    __shared__ int lds[kBlock / 64];
+    __shared__ int sbase;
     if (d.ctrl->halt) return;  // uniform over the grid: no thread reaches the barrier
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    bool take = false;
-    if (t < d.m) {
-        int c = d.classes[t], lt = d.low[t];
-        if (d.clear[t]) {
-            lt = -1;
-            c = 1;
-            d.low[t] = -1;
-            d.classes[t] = 1;
-            d.clear[t] = 0;
+    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
+    bool take[kItems];
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        const int t = base + k;
+        take[k] = false;
+        if (t < d.m) {
+            int c = d.classes[t], lt = d.low[t];
+            if (d.clear[t]) {
+                lt = -1;
+                c = 1;
+                d.low[t] = -1;
+                d.classes[t] = 1;
+                d.clear[t] = 0;
+            }
+            d.first[t] = kIntMax;
+            if (c == 0 && lt > -1) {
+                const int p = d.arglow[lt];
+                take[k] = p > -1 && p < t;
+            }
         }
-        d.first[t] = kIntMax;
-        if (c == 0 && lt > -1) {
-            int p = d.arglow[lt];
-            take = p > -1 && p < t;
-        }
+        cnt += take[k] ? 1 : 0;
     }
-    block_append(d.work, &d.ctrl->nwork, t, take);
+    int tot;
+    const int ex = block_exclusive_scan(cnt, SumOp<int>(), 0, lds, &tot);
+    if (threadIdx.x == 0) sbase = tot ? atomicAdd(&d.ctrl->nwork, tot) : 0;
+    __syncthreads();
+    int o = sbase + ex;
+#pragma unroll
+    for (int k = 0; k < kItems; k++)
+        if (take[k]) d.work[o++] = base + k;
 }
 
 // Phase 0 and its clears; k_apply1 also resets first[] for Phase I (the
 // worklist it builds is dropped by the first k_iter_begin).
 void launch_phase0(const Dev &d, hipStream_t s) {
     int gm = (d.m + kBlock - 1) / kBlock;
+    if (!d.m) return;
     k_phase0<<<gm, kBlock, 0, s>>>(d);
-    k_apply1<<<gm, kBlock, 0, s>>>(d);
+    k_apply1<<<d.ntiles, kBlock, 0, s>>>(d);
 }
 
 void launch_apply1(const Dev &d, hipStream_t s) {
-    int gm = (d.m + kBlock - 1) / kBlock;
-    k_apply1<<<gm, kBlock, 0, s>>>(d);
+    if (d.ntiles) k_apply1<<<d.ntiles, kBlock, 0, s>>>(d);
 }
 
 // ---------------------------------------------------------------------------
@@ -615,9 +647,10 @@ __global__ void __launch_bounds__(kBlock) k_tile(Dev d) {
 // Convergence test (is_reduced, pms_functions.cu:7-27: no non-negative low
 // occurs twice, from k_tile's flag; the first iteration always runs, as in
 // pms.cu:39-44), the iteration cap, and the exclusive max-scan of the tile
-// maxima over tiles, per slot. One block.
+// maxima over tiles, per slot. One block; wave s scans slot s with no block
+// barrier: each lane takes a contiguous run of tiles, the wave scans the
+// runs' maxima, and each lane writes its run's prefixes.
 __global__ void __launch_bounds__(1024) k_tilescan(Dev d) {
-    __shared__ int lds[16];
     __shared__ int s_halt;
     if (threadIdx.x == 0) {
         Ctrl *c = d.ctrl;
@@ -633,16 +666,28 @@ __global__ void __launch_bounds__(1024) k_tilescan(Dev d) {
     __syncthreads();
     if (s_halt) return;
     const int nslots = d.ctrl->nslots;
-    for (int s = 0; s < nslots && s < kMaxSlots; s++) {
-        int running = -1;
-        for (int c0 = 0; c0 < d.ntiles; c0 += blockDim.x) {
-            int t = c0 + threadIdx.x;
-            int v = t < d.ntiles ? d.tile_max[t * kMaxSlots + s] : -1;
-            int tot;
-            int ex = block_exclusive_scan(v, MaxOp<int>(), -1, lds, &tot);
-            if (t < d.ntiles) d.tile_pre[t * kMaxSlots + s] = ex > running ? ex : running;
-            running = tot > running ? tot : running;
-        }
+    const int s = threadIdx.x >> 6, lane = lane_id();
+    if (s >= nslots || s >= kMaxSlots) return;
+    const int per = (d.ntiles + 63) / 64;
+    const int t0 = lane * per;
+    const int t1 = t0 + per < d.ntiles ? t0 + per : d.ntiles;
+    int run = -1;
+    for (int t = t0; t < t1; t++) {
+        const int v = d.tile_max[t * kMaxSlots + s];
+        run = v > run ? v : run;
+    }
+    int incl = run;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int x = __shfl_up(incl, o, 64);
+        if (lane >= o && x > incl) incl = x;
+    }
+    int pre = __shfl_up(incl, 1, 64);
+    if (lane == 0) pre = -1;
+    for (int t = t0; t < t1; t++) {
+        d.tile_pre[t * kMaxSlots + s] = pre;
+        const int v = d.tile_max[t * kMaxSlots + s];
+        pre = v > pre ? v : pre;
     }
 }
 
