@@ -250,18 +250,33 @@ __device__ __forceinline__ int lower_bound_i(const int *a, int n, int key) {
 // COO start of the column goes to off2 until the pool layout is known.
 // A wave covers 63 columns: lane l searches the start of column 63w + l, and
 // lane 63 that of the next wave's first column, so every column's end comes
-// from its right neighbour lane and each lane searches once.
+// from its right neighbour lane and each lane searches once. The longest
+// column reaches Ctrl::maxlen with one atomic per block: one per lane on that
+// single address took 23 us of a Torus-100 setup.
 __global__ void k_init_cols(Dev d, const int *__restrict__ rows1, const int *__restrict__ cols1) {
     const int lane = lane_id();
     const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long jl = w * 63 + lane;
     const int key = jl < d.m ? (int)jl + 1 : d.m + 1;
+    __shared__ int s_max;
+    if (threadIdx.x == 0) s_max = 0;
+    __syncthreads();
     const int lo = lower_bound_i(cols1, d.nnz, key);
     int hi = __shfl_down(lo, 1, 64);
-    if (lane == 63 || jl >= d.m) return;
-    const int j = (int)jl;
+    const bool live = lane != 63 && jl < d.m;
     if (hi < lo) hi = lo;  // unsorted input; flagged by k_entries
-    int n = hi - lo;
+    const int n = live ? hi - lo : 0;
+    int wmax = n;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int x = __shfl_xor(wmax, o, 64);
+        wmax = x > wmax ? x : wmax;
+    }
+    if (lane == 0 && wmax) atomicMax(&s_max, wmax);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_max) atomicMax(&d.ctrl->maxlen, s_max);
+    if (!live) return;
+    const int j = (int)jl;
     d.off2[j] = (unsigned)lo;
     d.len[j] = n;
     int l = n ? rows1[hi - 1] - 1 : -1;
@@ -276,7 +291,6 @@ __global__ void k_init_cols(Dev d, const int *__restrict__ rows1, const int *__r
     d.clear[j] = 0;
     d.newpiv[j] = -1;
     d.pivdesc[j] = make_int4(-1, 0, 0, 0);
-    if (n) atomicMax(&d.ctrl->maxlen, n);
 }
 
 static int init_cols_blocks(int m) {
@@ -1173,12 +1187,30 @@ __global__ void __launch_bounds__(kBlock) k_phase2_group(Dev d) {
         ctr.nel += __shfl_down(ctr.nel, o, 64);
         ctr.spills += __shfl_down(ctr.spills, o, 64);
     }
+    // then per block: the counters live on one address each, where every
+    // atomic waits for the one before
+    __shared__ unsigned long long s_add[kBlock / 64], s_el[kBlock / 64];
+    __shared__ int s_sp[kBlock / 64];
+    const int wid = threadIdx.x >> 6;
     if (lane_id() == 0) {
-        if (ctr.nadd) {
-            atomicAdd(&d.ctrl->additions, ctr.nadd);
-            atomicAdd(&d.ctrl->merge_elems, ctr.nel);
+        s_add[wid] = ctr.nadd;
+        s_el[wid] = ctr.nel;
+        s_sp[wid] = ctr.spills;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, e = 0;
+        int sp = 0;
+        for (int k = 0; k < kBlock / 64; k++) {
+            a += s_add[k];
+            e += s_el[k];
+            sp += s_sp[k];
         }
-        if (ctr.spills) atomicAdd(&d.ctrl->spills, ctr.spills);
+        if (a) {
+            atomicAdd(&d.ctrl->additions, a);
+            atomicAdd(&d.ctrl->merge_elems, e);
+        }
+        if (sp) atomicAdd(&d.ctrl->spills, sp);
     }
 }
 
