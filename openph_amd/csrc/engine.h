@@ -16,6 +16,8 @@ constexpr int kLongBlockLanes = 1024;  // lanes of the block kernels (long colum
 
 // Device-side scalars shared by all kernels of one reduction. Lives in
 // device memory; the host reads it back with one small copy.
+constexpr int kStripes = 16;
+
 struct Ctrl {
     unsigned long long pool_top;   // next free pool entry
     unsigned long long pool_cap;   // pool capacity (entries)
@@ -46,6 +48,12 @@ struct Ctrl {
     int trk_linf;                  // max_j |low[j] - low_true[j]|
     int trk_unreduced;             // #{j : classes[j] == 0}
     long long trk_ess;             // #{j : ess[j] == 1}
+    // Phase II blocks add their counters here, block b into stripe b % 16,
+    // and k_iter_end folds the stripes into additions / merge_elems / spills:
+    // device-scope atomics on one address wait for each other
+    unsigned long long st_add[kStripes];
+    unsigned long long st_el[kStripes];
+    int st_spill[kStripes];
 };
 
 // Why the iteration kernels stopped (Ctrl::halt). Every iteration kernel

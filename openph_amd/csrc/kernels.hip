@@ -213,6 +213,33 @@ __global__ void k_ctrl_init(Ctrl *c, unsigned long long cap, int max_iters) {
         c->overflows = 0;
         c->prev_add = 0;
         c->pool_peak = 0;
+        for (int k = 0; k < kStripes; k++) {
+            c->st_add[k] = 0;
+            c->st_el[k] = 0;
+            c->st_spill[k] = 0;
+        }
+    }
+}
+
+// The Phase II counters of a block into its stripe.
+__device__ __forceinline__ void stripe_add(Ctrl *c, unsigned long long nadd, unsigned long long nel,
+                                           int spills) {
+    const int k = blockIdx.x % kStripes;
+    if (nadd) {
+        atomicAdd(&c->st_add[k], nadd);
+        atomicAdd(&c->st_el[k], nel);
+    }
+    if (spills) atomicAdd(&c->st_spill[k], spills);
+}
+
+__device__ __forceinline__ void fold_stripes(Ctrl *c) {
+    for (int k = 0; k < kStripes; k++) {
+        c->additions += c->st_add[k];
+        c->merge_elems += c->st_el[k];
+        c->spills += c->st_spill[k];
+        c->st_add[k] = 0;
+        c->st_el[k] = 0;
+        c->st_spill[k] = 0;
     }
 }
 
@@ -508,7 +535,9 @@ __global__ void k_iter_begin(Ctrl *c) {
 // out; two overflowing passes in a row without progress (the second right
 // after a compaction) mean the pool budget cannot hold the work.
 __global__ void k_iter_end(Ctrl *c) {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || c->halt) return;
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    fold_stripes(c);  // also after a halt: the counters of a pass that ran
+    if (c->halt) return;
     if (c->err) {
         c->halt = kHaltError;
         return;
@@ -1206,11 +1235,7 @@ __global__ void __launch_bounds__(kBlock) k_phase2_group(Dev d) {
             e += s_el[k];
             sp += s_sp[k];
         }
-        if (a) {
-            atomicAdd(&d.ctrl->additions, a);
-            atomicAdd(&d.ctrl->merge_elems, e);
-        }
-        if (sp) atomicAdd(&d.ctrl->spills, sp);
+        stripe_add(d.ctrl, a, e, sp);
     }
 }
 
@@ -1231,11 +1256,7 @@ __global__ void __launch_bounds__(kLongBlock) k_phase2_long(Dev d) {
         phase2_column<BlockTeam<kLongBlock>, kLongW, false>(t, d, d.longw[w], lds, ctr);
     if (threadIdx.x == 0) {
         if (ctr.ovf) d.ctrl->overflow = 1;
-        if (ctr.nadd) {
-            atomicAdd(&d.ctrl->additions, ctr.nadd);
-            atomicAdd(&d.ctrl->merge_elems, ctr.nel);
-        }
-        if (ctr.spills) atomicAdd(&d.ctrl->spills, ctr.spills);
+        stripe_add(d.ctrl, ctr.nadd, ctr.nel, ctr.spills);
     }
 }
 
