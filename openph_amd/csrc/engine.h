@@ -49,7 +49,8 @@ struct Ctrl {
     int trk_unreduced;             // #{j : classes[j] == 0}
     long long trk_ess;             // #{j : ess[j] == 1}
     // Phase II blocks add their counters here, block b into stripe b % 16,
-    // and k_iter_end folds the stripes into additions / merge_elems / spills:
+    // and the iteration's end (in k_apply2) folds the stripes into additions
+    // / merge_elems / spills:
     // device-scope atomics on one address wait for each other
     unsigned long long st_add[kStripes];
     unsigned long long st_el[kStripes];
@@ -112,9 +113,6 @@ void launch_phase0(const Dev &d, hipStream_t s);
 // does this in k_first; the serial algorithms use it to clear the overflow
 // flag before they resume.
 void launch_iter_begin(const Dev &d, hipStream_t s);
-// End of an iteration: progress, stall, overflow and pool-fill checks that
-// set Ctrl::halt.
-void launch_iter_end(const Dev &d, hipStream_t s);
 void launch_set_halt(const Dev &d, int value, hipStream_t s);
 // Device clock (wall_clock64) of the current iteration into
 // stamps[(2 * (iter - 1) + which) % cap], unless halted.
@@ -135,6 +133,9 @@ void launch_batch_local_low(int *low1, const int *moff, int count, int m, hipStr
 bool phase2_shape_ok(int lanes, int wlds);
 int phase2_default_lds(int lanes);
 int phase2_blocks_per_cu(int lanes, int wlds);
+// After Phase II: pivots published, clears applied, ess and classes; its
+// first thread also ends the iteration (counters folded, Ctrl::halt set by
+// the progress, stall, overflow and pool-fill checks).
 void launch_apply2(const Dev &d, hipStream_t s);
 // The serial algorithms (standard, twist): one workgroup reduces the columns
 // order[0..n) in turn. On a full pool it stops with Ctrl::overflow = 1 and

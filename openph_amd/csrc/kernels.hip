@@ -528,14 +528,15 @@ __global__ void k_iter_begin(Ctrl *c) {
     iter_begin(c);
 }
 
-// After Phase II and its bookkeeping. While lows repeat, pms marks a
-// column or adds one in every iteration (the first
-// occurrence of the largest repeated low is marked, or its repeats are
-// reduced), so an iteration without either is a fault unless the pool ran
-// out; two overflowing passes in a row without progress (the second right
-// after a compaction) mean the pool budget cannot hold the work.
-__global__ void k_iter_end(Ctrl *c) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// The end of an iteration, after Phase II: fold the counters and set
+// Ctrl::halt. While lows repeat, pms marks a column or adds one in every
+// iteration (the first occurrence of the largest repeated low is marked, or
+// its repeats are reduced), so an iteration without either is a fault unless
+// the pool ran out; two overflowing passes in a row without progress (the
+// second right after a compaction) mean the pool budget cannot hold the work.
+// Run by one thread of k_apply2: it reads only what the earlier kernels of
+// the iteration wrote, and k_apply2 writes none of it.
+__device__ void iter_end(Ctrl *c) {
     fold_stripes(c);  // also after a halt: the counters of a pass that ran
     if (c->halt) return;
     if (c->err) {
@@ -577,7 +578,6 @@ void launch_stamp(const Dev &d, unsigned long long *stamps, int which, int cap, 
 }
 
 void launch_iter_begin(const Dev &d, hipStream_t s) { k_iter_begin<<<1, 64, 0, s>>>(d.ctrl); }
-void launch_iter_end(const Dev &d, hipStream_t s) { k_iter_end<<<1, 64, 0, s>>>(d.ctrl); }
 void launch_set_halt(const Dev &d, int value, hipStream_t s) {
     k_set_halt<<<1, 64, 0, s>>>(d.ctrl, value);
 }
@@ -1704,9 +1704,12 @@ void launch_ess_at_lows(const Dev &d, hipStream_t s) {
 
 // After Phase II: publish new pivots, clear_positives, ess_hat
 // (kernels.cu:113-124) and update_classes (kernels.cu:93-101). Every thread
-// touches its own column first, so one pass suffices.
+// touches its own column first, so one pass suffices. The kernel does not
+// stop on Ctrl::halt, which its first thread sets for the next iteration:
+// after a halted iteration nothing is pending (newpiv and clear were reset
+// by the last pass) and the pass changes nothing.
 __global__ void __launch_bounds__(kBlock) k_apply2(Dev d) {
-    if (d.ctrl->halt) return;  // uniform over the grid
+    if (blockIdx.x == 0 && threadIdx.x == 0) iter_end(d.ctrl);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < d.m) {
         int np = d.newpiv[t];

@@ -191,8 +191,7 @@ static int enqueue_iteration(const Dev &d, const Shape &sh, hipStream_t s, hipEv
     launch_phase2(d, sh.lanes, sh.wlds, sh.grid, sh.long_grid, s);
     if (stamps) launch_stamp(d, stamps, 1, kStampRing, s);
     if (e1) OPH_CHECK(hipEventRecord(e1, s));
-    launch_apply2(d, s);
-    launch_iter_end(d, s);
+    launch_apply2(d, s);  // ends the iteration: Ctrl::halt for the next
     return OPENPH_OK;
 }
 
@@ -453,7 +452,6 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
             }
             p2_launches++;
             launch_apply2(d, s);
-            launch_iter_end(d, s);
             if (iter_times && iter <= m) OPH_CHECK(hipEventRecord(evs[2 * iter + 1], s));
             if (tracking) {
                 if ((rc = track_state(c, tr, iter, s, m, ess_true))) return rc;
