@@ -46,7 +46,15 @@ extern "C" {
  *
  *   algstr      "pms" | "standard" | "twist" | "ph_row" |
  *               "standard_parallel" | "twist_parallel" | "ph_row_parallel"
- *               (src/cuda/algorithms/algorithm_factory.cu:10-148)
+ *               (src/cuda/algorithms/algorithm_factory.cu:10-148). Each
+ *               runs its own algorithm on the device: pms iterations;
+ *               standard and twist one column at a time in the reference's
+ *               order (a _parallel name runs the same order); ph_row and
+ *               ph_row_parallel one row at a time, from the last. low is
+ *               the same for all on a boundary matrix; ess is each
+ *               algorithm's own estimate, as in the reference. For all but
+ *               pms, num_iters is steps + 1, time_track[i] the device time
+ *               of step i, and the err_* trackers are left as given.
  *   col_width   accepted and ignored: columns live in an adaptive device
  *               pool, there is no fixed p = col_width * max_nnz.
  *   h_low_true  1-based true low array (0 = none) used only by the
