@@ -981,16 +981,33 @@ struct WaveTeam {
     __device__ void fence() const { __threadfence_block(); }
     __device__ int scan(int v, int &total) const {
         int incl = v;
+        if constexpr (G <= 16) {
+            // a group lies inside one 16-lane row: row shifts (DPP) instead
+            // of LDS permutes; a lane adds only what comes from its group
+            incl += dpp_up<1>(incl, gl);
+            if constexpr (G > 2) incl += dpp_up<2>(incl, gl);
+            if constexpr (G > 4) incl += dpp_up<4>(incl, gl);
+            if constexpr (G > 8) incl += dpp_up<8>(incl, gl);
+        } else {
 #pragma unroll
-        for (int o = 1; o < G; o <<= 1) {
-            const int t = __shfl_up(incl, o, G);
-            if (gl >= o) incl += t;
+            for (int o = 1; o < G; o <<= 1) {
+                const int t = __shfl_up(incl, o, G);
+                if (gl >= o) incl += t;
+            }
         }
         total = __shfl(incl, G - 1, G);
         return incl - v;
     }
     __device__ unsigned long long bcast(unsigned long long x) const {
         return __shfl(x, gw * G, 64);
+    }
+
+  private:
+    // x of the lane O below, 0 for the group's first O lanes (row_shr:O)
+    template <int O>
+    __device__ static int dpp_up(int x, int gl) {
+        const int t = __builtin_amdgcn_update_dpp(0, x, 0x110 + O, 0xF, 0xF, false);
+        return gl >= O ? t : 0;
     }
 };
 
@@ -1094,9 +1111,13 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
         const unsigned aoff = (unsigned)pd.y;
         int nc, nl;
         if (!glob && na + clen <= W) {
-            if (gl == 0) d.ess[lj] = 0;
             for (int i = gl; i < na; i += G) sA[i] = d.pool[aoff + i];
             t.sync();
+            // Stores go after the loads they would otherwise delay: a wait
+            // for a load also waits for every older store. Every lane of the
+            // team stores (the same value to the same address), so that no
+            // branch hides the store from the compiler's wait counting.
+            d.ess[lj] = 0;
             nc = team_xor_lds(t, sA, na, sB, clen, sC);
             t.sync();
             int *tmp = sB;  // the sum is in sA's buffer
@@ -1136,8 +1157,8 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
         }
         lj = nl;
         if (lj < 0) break;
-        if (gl == 0) d.clear[lj] = 1;
         pd = d.pivdesc[lj];  // .w = left[lj]: one load for the rule and the pivot
+        d.clear[lj] = 1;
         if (pd.w == j) {
             becomes_pivot = true;
             break;
