@@ -968,6 +968,32 @@ __device__ __forceinline__ int mp_walk(const int *a, int na, const int *b, int n
     return cnt;
 }
 
+// mp_walk for operands in LDS, without branches: one LDS read per step from
+// whichever list the step consumed (no flat addressing: both are LDS).
+__device__ __forceinline__ int mp_walk_lds(const int *a, int na, const int *b, int nb, int d0,
+                                           int d1, int *c) {
+    int i = mp_split(a, na, b, nb, d0), k = d0 - i;
+    int av = i < na ? a[i] : kIntMax;
+    int bv = k < nb ? b[k] : kIntMax;
+    int prev_a = i > 0 ? a[i - 1] : -1;
+    int cnt = 0;
+    for (int d = d0; d < d1; d++) {
+        const bool ta = av <= bv;
+        const int v = ta ? av : bv;
+        const bool keep = ta ? bv != av : prev_a != bv;
+        prev_a = ta ? av : prev_a;
+        i += ta ? 1 : 0;
+        k += ta ? 0 : 1;
+        const int idx = ta ? i : k;
+        const int nv = idx < (ta ? na : nb) ? (ta ? a : b)[idx] : kIntMax;
+        av = ta ? nv : av;
+        bv = ta ? bv : nv;
+        if (keep) c[cnt] = v;
+        cnt += keep ? 1 : 0;
+    }
+    return cnt;
+}
+
 // A team merges one column: G lanes of one wave (WaveTeam) or a whole block
 // (BlockTeam). It provides its lane index and size, a barrier, an exclusive
 // scan with total, a broadcast from its leader and pool allocation.
@@ -1060,11 +1086,21 @@ __device__ int team_xor_lds(const Team &t, int *a, int na, const int *b, int nb,
     const int l = t.lane();
     const int d0 = l * K < n ? l * K : n;
     const int d1 = d0 + K < n ? d0 + K : n;
-    const int cnt = mp_walk<true>(a, na, b, nb, d0, d1, c + d0);
+    const int cnt = mp_walk_lds(a, na, b, nb, d0, d1, c + d0);
     int total;
     const int ex = t.scan(cnt, total);  // every lane's walk is over after this
     t.sync();
-    for (int q = 0; q < cnt; q++) a[ex + q] = c[d0 + q];
+    // four reads, then four writes: one LDS round trip per four survivors
+    for (int q = 0; q < cnt; q += 4) {
+        const int x0 = c[d0 + q];
+        const int x1 = q + 1 < cnt ? c[d0 + q + 1] : 0;
+        const int x2 = q + 2 < cnt ? c[d0 + q + 2] : 0;
+        const int x3 = q + 3 < cnt ? c[d0 + q + 3] : 0;
+        a[ex + q] = x0;
+        if (q + 1 < cnt) a[ex + q + 1] = x1;
+        if (q + 2 < cnt) a[ex + q + 2] = x2;
+        if (q + 3 < cnt) a[ex + q + 3] = x3;
+    }
     return total;
 }
 
