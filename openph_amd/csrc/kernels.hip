@@ -734,11 +734,41 @@ __global__ void __launch_bounds__(1024) k_tilescan(Dev d) {
     }
 }
 
+// FUSED (few tiles): k_tilescan's work done here, by every block from the
+// same inputs: the loop's exit (block 0 writes it) and the maximum over the
+// earlier tiles, read by each block itself. One launch less per iteration.
+template <bool FUSED>
 __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
     __shared__ int lds[kBlock / 64];
     __shared__ int exl[kMaxSlots][kBlock];
-    if (d.ctrl->halt) return;
+    __shared__ int s_tp[kMaxSlots];
     const int nslots = d.ctrl->nslots;
+    if (FUSED) {
+        const Ctrl *c = d.ctrl;
+        int h = c->halt;
+        if (!h) {
+            if (c->err) h = kHaltError;
+            else if (c->iter > 1 && !c->any_dup) h = kHaltConverged;
+            else if (c->iter > c->max_iters) h = kHaltIters;
+        }
+        if (h) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->halt = h;
+            return;
+        }
+        for (int sl = 0; sl < nslots && sl < kMaxSlots; sl++) {
+            int mx = -1;
+            for (int t = threadIdx.x; t < (int)blockIdx.x; t += kBlock) {
+                const int v = d.tile_max[t * kMaxSlots + sl];
+                mx = v > mx ? v : mx;
+            }
+            int tot;
+            (void)block_exclusive_scan(mx, MaxOp<int>(), -1, lds, &tot);
+            if (threadIdx.x == 0) s_tp[sl] = tot;
+        }
+        __syncthreads();
+    } else if (d.ctrl->halt) {
+        return;
+    }
     const int base = blockIdx.x * kTile + threadIdx.x * kItems;
     int lo[kItems], sl[kItems], dup[kItems], fst[kItems];
 #pragma unroll
@@ -763,11 +793,12 @@ __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
                 if (sl[k] == s && dup[k] > agg) agg = dup[k];
             int tot;
             int ex = block_exclusive_scan(agg, MaxOp<int>(), -1, lds, &tot);
-            int tp = d.tile_pre[blockIdx.x * kMaxSlots + s];
+            int tp = FUSED ? s_tp[s] : d.tile_pre[blockIdx.x * kMaxSlots + s];
             exl[s][threadIdx.x] = ex > tp ? ex : tp;
         }
     }
     __syncthreads();
+    int marked = 0;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
         int j = base + k;
@@ -778,7 +809,7 @@ __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
             if (sl[q] == sl[k] && dup[q] > c) c = dup[q];
         if (d.classes[j] == 0 && lo[k] > c) {
             int L = lo[k];
-            atomicAdd(&d.ctrl->marks, 1);
+            marked++;
             publish_pivot(d, L, j);
             d.classes[j] = -1;
             d.clear[L] = 1;
@@ -786,6 +817,10 @@ __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
             d.ess[j] = 0;
         }
     }
+    // one atomic per block on the shared counter
+    int tot;
+    (void)block_exclusive_scan(marked, SumOp<int>(), 0, lds, &tot);
+    if (threadIdx.x == 0 && tot) atomicAdd(&d.ctrl->marks, tot);
 }
 
 void launch_phase1_detect(const Dev &d, hipStream_t s) {
@@ -793,9 +828,17 @@ void launch_phase1_detect(const Dev &d, hipStream_t s) {
     k_tile<<<d.ntiles, kBlock, 0, s>>>(d);
 }
 
+// Up to this many tiles each k_mark block reads the earlier tiles' maxima
+// itself (at most 2 loads per thread and slot) instead of a k_tilescan pass.
+constexpr int kFuseTiles = 512;
+
 void launch_phase1_mark(const Dev &d, hipStream_t s) {
-    k_tilescan<<<1, 1024, 0, s>>>(d);
-    k_mark<<<d.ntiles, kBlock, 0, s>>>(d);
+    if (d.ntiles <= kFuseTiles) {
+        k_mark<true><<<d.ntiles, kBlock, 0, s>>>(d);
+    } else {
+        k_tilescan<<<1, 1024, 0, s>>>(d);
+        k_mark<false><<<d.ntiles, kBlock, 0, s>>>(d);
+    }
 }
 
 // ---------------------------------------------------------------------------
