@@ -86,6 +86,44 @@ def test_graph_schedule_is_used():
     assert st.host_reads == 1
 
 
+def test_graph_replays_interleaved_with_other_work():
+    """One cached graph replayed several times on device buffers, with other
+    reductions (other sizes, other schedules) and eager torch work between
+    the replays: every replay equals the eager (sync) result."""
+    import torch
+    say("graph replays")
+    dev = torch.device("cuda:0")
+    b = _gen("torus100")
+    f = np.load(os.path.join(GOLD, "torus100.npz"))
+    snap = INDEX["torus100"]["pms_snapshot"]
+    r, c = b.coo1()
+    R = torch.from_numpy(r).to(dev)
+    Cc = torch.from_numpy(c).to(dev)
+    low = torch.empty(b.m, dtype=torch.int32, device=dev)
+    ess = torch.empty_like(low)
+    st = openph_amd.pms_device(R, Cc, b.m, low, ess, options=_opt(O.SYNC))
+    torch.cuda.synchronize()
+    eager_low, eager_ess = low.cpu().numpy().copy(), ess.cpu().numpy().copy()
+    assert np.array_equal(eager_low, f["low"] + 1) and np.array_equal(eager_ess, f["ess"])
+    other = C.vr40()
+    ro, co = other.coo1()
+    fo = np.load(os.path.join(GOLD, "vr40.npz"))
+    for k in range(4):
+        low.fill_(-7)
+        ess.fill_(-7)
+        st = openph_amd.pms_device(R, Cc, b.m, low, ess, options=_opt(O.GRAPH))
+        x = torch.arange(1 << 20, device=dev, dtype=torch.float32).sum()  # eager work
+        torch.cuda.synchronize()
+        assert float(x) > 0
+        assert st.schedule == O.GRAPH
+        assert st.num_iters == snap["num_iters"] and st.additions == snap["additions"]
+        assert np.array_equal(low.cpu().numpy(), eager_low), k
+        assert np.array_equal(ess.cpu().numpy(), eager_ess), k
+        # another size on the same workspace in between
+        lo2, es2, _ = openph_amd.reduce(other, "pms", _opt(O.QUEUED if k % 2 else O.GRAPH))
+        assert np.array_equal(lo2, fo["low"]) and np.array_equal(es2, fo["ess"])
+
+
 @pytest.mark.parametrize("schedule", [O.SYNC, O.QUEUED, O.GRAPH])
 def test_schedules_compact_and_stay_exact(schedule):
     say("small pool", schedule)
