@@ -1370,6 +1370,8 @@ static Phase2Kernel phase2_kernel(int lanes, int wlds) {
     OPH_SHAPE(4, 64)
     OPH_SHAPE(8, 32)
     OPH_SHAPE(8, 64)
+    OPH_SHAPE(8, 80)
+    OPH_SHAPE(8, 96)
     OPH_SHAPE(8, 128)
     OPH_SHAPE(16, 64)
     OPH_SHAPE(16, 128)
@@ -1389,7 +1391,7 @@ bool phase2_shape_ok(int lanes, int wlds) {
 int phase2_default_lds(int lanes) {
     switch (lanes) {
         case 4: return 64;
-        case 8: return 64;
+        case 8: return 80;
         case 16: return 128;
         case 32: return 512;
         case 64: return 1024;
