@@ -1147,6 +1147,34 @@ __device__ int team_xor_lds(const Team &t, int *a, int na, const int *b, int nb,
     return total;
 }
 
+// The same sum with the result in b's buffer: survivors go to provisional
+// slots of c at their first diagonal, and after the scan (every walk is over)
+// to their places in b. a may be in LDS (LDS_A) or in the pool.
+template <bool LDS_A, class Team>
+__device__ int team_xor_into_b(const Team &t, const int *a, int na, int *b, int nb, int *c) {
+    const int n = na + nb;
+    const int K = (n + Team::size - 1) / Team::size;
+    const int l = t.lane();
+    const int d0 = l * K < n ? l * K : n;
+    const int d1 = d0 + K < n ? d0 + K : n;
+    const int cnt = LDS_A ? mp_walk_lds(a, na, b, nb, d0, d1, c + d0)
+                          : mp_walk<true>(a, na, b, nb, d0, d1, c + d0);
+    int total;
+    const int ex = t.scan(cnt, total);
+    t.sync();
+    for (int q = 0; q < cnt; q += 4) {
+        const int x0 = c[d0 + q];
+        const int x1 = q + 1 < cnt ? c[d0 + q + 1] : 0;
+        const int x2 = q + 2 < cnt ? c[d0 + q + 2] : 0;
+        const int x3 = q + 3 < cnt ? c[d0 + q + 3] : 0;
+        b[ex + q] = x0;
+        if (q + 1 < cnt) b[ex + q + 1] = x1;
+        if (q + 2 < cnt) b[ex + q + 2] = x2;
+        if (q + 3 < cnt) b[ex + q + 3] = x3;
+    }
+    return total;
+}
+
 // Allocate `n` pool entries by the team leader; ~0ull when the pool is full.
 template <class Team>
 __device__ __forceinline__ unsigned long long team_alloc(const Team &t, const Dev &d, unsigned n) {
@@ -1166,11 +1194,12 @@ struct Phase2Counters {
 
 // The Phase II chain of column j (pms_functions.cu:87-112) by one team. The
 // column's contents stay in LDS buffer sB (W entries) while they fit; a
-// pivot column is staged into sA and the sum goes to sC. With HANDOFF, a
-// column whose next sum may not fit W is committed as it stands and queued
-// on the long list for the block kernel, which continues the same chain;
-// without it, the team merges straight in the pool.
-template <class Team, int W, bool HANDOFF>
+// pivot column of at most WA entries is staged into sA (a longer one is read
+// from the pool), the survivors of the sum go to sC and then back to sB.
+// With HANDOFF, a column whose next sum may not fit W is committed as it
+// stands and queued on the long list for the block kernel, which continues
+// the same chain; without it, the team merges straight in the pool.
+template <class Team, int W, int WA, bool HANDOFF>
 __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phase2Counters &ctr) {
     const int gl = t.lane();
     constexpr int G = Team::size;
@@ -1178,7 +1207,7 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
     int4 pd = d.pivdesc[lj];
     unsigned coff = d.off[j];
     int clen = d.len[j], ccap = d.cap[j];
-    int *sA = buf, *sB = buf + W, *sC = buf + 2 * W;
+    int *sA = buf, *sB = buf + WA, *sC = buf + WA + W;
     bool glob = clen > W;  // contents in the pool instead of LDS
     bool handoff = HANDOFF && glob;
     if (!glob)
@@ -1190,18 +1219,21 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
         const unsigned aoff = (unsigned)pd.y;
         int nc, nl;
         if (!glob && na + clen <= W) {
-            for (int i = gl; i < na; i += G) sA[i] = d.pool[aoff + i];
-            t.sync();
+            const bool ain = WA >= W || na <= WA;
+            if (ain) {
+                for (int i = gl; i < na; i += G) sA[i] = d.pool[aoff + i];
+                t.sync();
+            }
             // Stores go after the loads they would otherwise delay: a wait
             // for a load also waits for every older store. Every lane of the
             // team stores (the same value to the same address), so that no
             // branch hides the store from the compiler's wait counting.
             d.ess[lj] = 0;
-            nc = team_xor_lds(t, sA, na, sB, clen, sC);
+            if (ain)
+                nc = team_xor_into_b<true>(t, sA, na, sB, clen, sC);
+            else
+                nc = team_xor_into_b<false>(t, d.pool + aoff, na, sB, clen, sC);
             t.sync();
-            int *tmp = sB;  // the sum is in sA's buffer
-            sB = sA;
-            sA = tmp;
             nl = nc ? sB[nc - 1] : -1;
         } else if (HANDOFF) {
             handoff = true;
@@ -1286,27 +1318,35 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
     t.sync();  // the buffers are reused by the team's next column
 }
 
-// Lane-group shapes: G lanes per column, W LDS entries per buffer (three
-// buffers per group), kBlock / G groups per block.
+// Lane-group shapes: G lanes per column, W LDS entries for the column and
+// for the sum, WA for the pivot, kBlock / G groups per block. WA = W but for
+// (8, 80), whose 16-entry pivot buffer (99.4 % of the pivots of a VR
+// 2-skeleton reduction fit, the others are read from the pool) brings a
+// block to 22 KB: seven blocks share a CU's 160 KB instead of five.
+constexpr int pivot_lds(int G, int W) { return G == 8 && W == 80 ? 16 : W; }
+
 template <int G, int W>
 struct GroupShape {
     static constexpr int NG = kBlock / G;
-    static constexpr int LDS_BYTES = NG * 3 * W * 4;
+    static constexpr int WA = pivot_lds(G, W);
+    static constexpr int PER = WA + 2 * W;  // LDS entries per group
+    static constexpr int LDS_BYTES = NG * PER * 4;
     static_assert(LDS_BYTES <= 64 * 1024, "static LDS limit");
 };
 
 template <int G, int W>
 __global__ void __launch_bounds__(kBlock) k_phase2_group(Dev d) {
-    constexpr int NG = GroupShape<G, W>::NG;
-    __shared__ int lds[NG * 3 * W];
+    using S = GroupShape<G, W>;
+    constexpr int NG = S::NG;
+    __shared__ __align__(16) int lds[NG * S::PER];
     const WaveTeam<G> t{(int)(threadIdx.x % G), (int)((threadIdx.x & 63) / G)};
     const int gid = threadIdx.x / G;
-    int *const buf = lds + gid * 3 * W;
+    int *const buf = lds + gid * S::PER;
     if (d.ctrl->halt) return;
     const int nwork = d.ctrl->nwork;
     Phase2Counters ctr;
     for (int w = blockIdx.x * NG + gid; w < nwork; w += gridDim.x * NG)
-        phase2_column<WaveTeam<G>, W, true>(t, d, d.work[w], buf, ctr);
+        phase2_column<WaveTeam<G>, W, S::WA, true>(t, d, d.work[w], buf, ctr);
     if (ctr.ovf && t.gl == 0) d.ctrl->overflow = 1;
     // per-group counters are identical on the group's lanes: count leaders only
     if (t.gl != 0) ctr = Phase2Counters();
@@ -1353,7 +1393,7 @@ __global__ void __launch_bounds__(kLongBlock) k_phase2_long(Dev d) {
     const int nlong = d.ctrl->nlong;
     Phase2Counters ctr;
     for (int w = blockIdx.x; w < nlong; w += gridDim.x)
-        phase2_column<BlockTeam<kLongBlock>, kLongW, false>(t, d, d.longw[w], lds, ctr);
+        phase2_column<BlockTeam<kLongBlock>, kLongW, kLongW, false>(t, d, d.longw[w], lds, ctr);
     if (threadIdx.x == 0) {
         if (ctr.ovf) d.ctrl->overflow = 1;
         stripe_add(d.ctrl, ctr.nadd, ctr.nel, ctr.spills);
