@@ -102,7 +102,7 @@ typedef struct openph_options {
     int slot_elems;          /* minimum pool segment per column (0 = 32)     */
     int group_lds;           /* LDS entries per lane-group buffer (0 = the
                                 default of lanes_per_column); the compiled
-                                shapes are {4: 32, 64}, {8: 32, 64, 80, 96, 128},
+                                shapes are {4: 32, 64}, {8: 32, 64, 72, 80, 96, 128},
                                 {16: 64, 128, 256}, {32: 256, 512},
                                 {64: 512, 1024}                              */
     int schedule;            /* how the host drives the pms iterations:

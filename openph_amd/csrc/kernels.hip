@@ -1012,13 +1012,25 @@ __device__ __forceinline__ int mp_walk(const int *a, int na, const int *b, int n
 }
 
 // mp_walk for operands in LDS, without branches: one LDS read per step from
-// whichever list the step consumed (no flat addressing: both are LDS).
+// whichever list the step consumed (no flat addressing: both are LDS). With
+// SENT, a[na] and b[nb] hold kIntMax, so no read is guarded, and every step
+// writes its element to c (a survivor's slot is the next one): the writes
+// stay inside the lane's own stretch of diagonals.
+template <bool SENT>
 __device__ __forceinline__ int mp_walk_lds(const int *a, int na, const int *b, int nb, int d0,
                                            int d1, int *c) {
     int i = mp_split(a, na, b, nb, d0), k = d0 - i;
-    int av = i < na ? a[i] : kIntMax;
-    int bv = k < nb ? b[k] : kIntMax;
-    int prev_a = i > 0 ? a[i - 1] : -1;
+    int av, bv, prev_a;
+    if (SENT) {
+        av = a[i];
+        bv = b[k];
+        prev_a = a[i > 0 ? i - 1 : 0];
+        prev_a = i > 0 ? prev_a : -1;
+    } else {
+        av = i < na ? a[i] : kIntMax;
+        bv = k < nb ? b[k] : kIntMax;
+        prev_a = i > 0 ? a[i - 1] : -1;
+    }
     int cnt = 0;
     for (int d = d0; d < d1; d++) {
         const bool ta = av <= bv;
@@ -1027,11 +1039,20 @@ __device__ __forceinline__ int mp_walk_lds(const int *a, int na, const int *b, i
         prev_a = ta ? av : prev_a;
         i += ta ? 1 : 0;
         k += ta ? 0 : 1;
-        const int idx = ta ? i : k;
-        const int nv = idx < (ta ? na : nb) ? (ta ? a : b)[idx] : kIntMax;
+        int nv;
+        if (SENT) {
+            nv = (ta ? a + i : b + k)[0];
+        } else {
+            const int idx = ta ? i : k;
+            nv = idx < (ta ? na : nb) ? (ta ? a : b)[idx] : kIntMax;
+        }
         av = ta ? nv : av;
         bv = ta ? bv : nv;
-        if (keep) c[cnt] = v;
+        if (SENT) {
+            c[cnt] = v;
+        } else if (keep) {
+            c[cnt] = v;
+        }
         cnt += keep ? 1 : 0;
     }
     return cnt;
@@ -1129,7 +1150,7 @@ __device__ int team_xor_lds(const Team &t, int *a, int na, const int *b, int nb,
     const int l = t.lane();
     const int d0 = l * K < n ? l * K : n;
     const int d1 = d0 + K < n ? d0 + K : n;
-    const int cnt = mp_walk_lds(a, na, b, nb, d0, d1, c + d0);
+    const int cnt = mp_walk_lds<false>(a, na, b, nb, d0, d1, c + d0);
     int total;
     const int ex = t.scan(cnt, total);  // every lane's walk is over after this
     t.sync();
@@ -1149,29 +1170,37 @@ __device__ int team_xor_lds(const Team &t, int *a, int na, const int *b, int nb,
 
 // The same sum with the result in b's buffer: survivors go to provisional
 // slots of c at their first diagonal, and after the scan (every walk is over)
-// to their places in b. a may be in LDS (LDS_A) or in the pool.
+// to their places in b, followed by a new sentinel. a may be in LDS (LDS_A,
+// with its sentinel) or in the pool; b[nb] holds the sentinel kIntMax, so
+// n + 1 entries of b's buffer are used.
 template <bool LDS_A, class Team>
-__device__ int team_xor_into_b(const Team &t, const int *a, int na, int *b, int nb, int *c) {
+__device__ int team_xor_into_b(const Team &t, const int *a, int na, int *b, int nb, int *c,
+                               int spare) {
     const int n = na + nb;
-    const int K = (n + Team::size - 1) / Team::size;
+    const int K = (int)(((unsigned)n + Team::size - 1) / Team::size);
     const int l = t.lane();
     const int d0 = l * K < n ? l * K : n;
     const int d1 = d0 + K < n ? d0 + K : n;
-    const int cnt = LDS_A ? mp_walk_lds(a, na, b, nb, d0, d1, c + d0)
+    const int cnt = LDS_A ? mp_walk_lds<true>(a, na, b, nb, d0, d1, c + d0)
                           : mp_walk<true>(a, na, b, nb, d0, d1, c + d0);
     int total;
     const int ex = t.scan(cnt, total);
     t.sync();
+    // four reads, then four writes, without branches: reads past the lane's
+    // survivors are clamped to c[spare] and their writes go there (spare >= n
+    // is a slot of c that no walk uses)
     for (int q = 0; q < cnt; q += 4) {
-        const int x0 = c[d0 + q];
-        const int x1 = q + 1 < cnt ? c[d0 + q + 1] : 0;
-        const int x2 = q + 2 < cnt ? c[d0 + q + 2] : 0;
-        const int x3 = q + 3 < cnt ? c[d0 + q + 3] : 0;
+        const int s0 = d0 + q;
+        const int x0 = c[s0];
+        const int x1 = c[q + 1 < cnt ? s0 + 1 : spare];
+        const int x2 = c[q + 2 < cnt ? s0 + 2 : spare];
+        const int x3 = c[q + 3 < cnt ? s0 + 3 : spare];
         b[ex + q] = x0;
-        if (q + 1 < cnt) b[ex + q + 1] = x1;
-        if (q + 2 < cnt) b[ex + q + 2] = x2;
-        if (q + 3 < cnt) b[ex + q + 3] = x3;
+        (q + 1 < cnt ? b + ex + q + 1 : c + spare)[0] = x1;
+        (q + 2 < cnt ? b + ex + q + 2 : c + spare)[0] = x2;
+        (q + 3 < cnt ? b + ex + q + 3 : c + spare)[0] = x3;
     }
+    b[total] = kIntMax;  // every lane: the same value
     return total;
 }
 
@@ -1208,20 +1237,25 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
     unsigned coff = d.off[j];
     int clen = d.len[j], ccap = d.cap[j];
     int *sA = buf, *sB = buf + WA, *sC = buf + WA + W;
-    bool glob = clen > W;  // contents in the pool instead of LDS
+    // contents in the pool instead of LDS; in LDS they end with the sentinel
+    // kIntMax, so a column there holds fewer than W entries
+    bool glob = clen >= W;
     bool handoff = HANDOFF && glob;
-    if (!glob)
+    if (!glob) {
         for (int i = gl; i < clen; i += G) sB[i] = d.pool[coff + i];
+        sB[clen] = kIntMax;
+    }
     t.sync();
     bool dirty = false, becomes_pivot = false, lost = false;
     while (!handoff && pd.x > -1 && pd.x < j) {
         const int na = pd.z;
         const unsigned aoff = (unsigned)pd.y;
         int nc, nl;
-        if (!glob && na + clen <= W) {
-            const bool ain = WA >= W || na <= WA;
+        if (!glob && na + clen < W) {
+            const bool ain = na < WA;
             if (ain) {
                 for (int i = gl; i < na; i += G) sA[i] = d.pool[aoff + i];
+                sA[na] = kIntMax;
                 t.sync();
             }
             // Stores go after the loads they would otherwise delay: a wait
@@ -1230,9 +1264,9 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
             // branch hides the store from the compiler's wait counting.
             d.ess[lj] = 0;
             if (ain)
-                nc = team_xor_into_b<true>(t, sA, na, sB, clen, sC);
+                nc = team_xor_into_b<true>(t, sA, na, sB, clen, sC, W - 1);
             else
-                nc = team_xor_into_b<false>(t, d.pool + aoff, na, sB, clen, sC);
+                nc = team_xor_into_b<false>(t, d.pool + aoff, na, sB, clen, sC, W - 1);
             t.sync();
             nl = nc ? sB[nc - 1] : -1;
         } else if (HANDOFF) {
@@ -1320,10 +1354,14 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
 
 // Lane-group shapes: G lanes per column, W LDS entries for the column and
 // for the sum, WA for the pivot, kBlock / G groups per block. WA = W but for
-// (8, 80), whose 16-entry pivot buffer (99.4 % of the pivots of a VR
-// 2-skeleton reduction fit, the others are read from the pool) brings a
-// block to 22 KB: seven blocks share a CU's 160 KB instead of five.
-constexpr int pivot_lds(int G, int W) { return G == 8 && W == 80 ? 16 : W; }
+// the simplicial shapes of 8 lanes. A 16-entry pivot buffer (99.4 % of the
+// pivots of a VR 2-skeleton reduction fit, the others are read from the
+// pool) brings an (8, 80) block to 22 KB: seven blocks share a CU's 160 KB
+// instead of five. (8, 72) with 12 entries and (8, 64) with 16 stay under
+// 20 KB: eight blocks, the most the kernel's registers allow.
+constexpr int pivot_lds(int G, int W) {
+    return G != 8 ? W : W == 80 ? 16 : W == 72 ? 12 : W == 64 ? 16 : W;
+}
 
 template <int G, int W>
 struct GroupShape {
@@ -1410,6 +1448,7 @@ static Phase2Kernel phase2_kernel(int lanes, int wlds) {
     OPH_SHAPE(4, 64)
     OPH_SHAPE(8, 32)
     OPH_SHAPE(8, 64)
+    OPH_SHAPE(8, 72)
     OPH_SHAPE(8, 80)
     OPH_SHAPE(8, 96)
     OPH_SHAPE(8, 128)

@@ -139,8 +139,9 @@ extern "C" int openph_autotune_batch_device(int count, const int *ms, const int 
     };
     // 1. Phase II shape
     static const int shapes[][2] = {{1, 0},    {4, 32},   {4, 64},   {8, 32},   {8, 64},
-                                    {8, 80},   {8, 96},   {8, 128},  {16, 64},  {16, 128},
-                                    {16, 256}, {32, 256}, {32, 512}, {64, 512}, {64, 1024}};
+                                    {8, 72},   {8, 80},   {8, 96},   {8, 128},  {16, 64},
+                                    {16, 128}, {16, 256}, {32, 256}, {32, 512}, {64, 512},
+                                    {64, 1024}};
     for (const auto &sh : shapes) {
         openph_options o = b0;
         o.lanes_per_column = sh[0];

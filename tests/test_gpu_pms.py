@@ -53,7 +53,7 @@ def test_pms_matches_golden(name):
     assert st.m == b.m
 
 
-SHAPES = [(1, 0), (4, 32), (4, 64), (8, 32), (8, 64), (8, 80), (8, 96), (8, 128), (16, 64),
+SHAPES = [(1, 0), (4, 32), (4, 64), (8, 32), (8, 64), (8, 72), (8, 80), (8, 96), (8, 128), (16, 64),
           (16, 128), (16, 256), (32, 256), (32, 512), (64, 512), (64, 1024)]
 
 

@@ -293,7 +293,7 @@ def test_autotune_batch_checks_every_candidate_and_returns_a_working_setting():
     low = torch.empty(M, dtype=torch.int32, device=dev)
     ess = torch.empty_like(low)
     best, rows_ = openph_amd.autotune_batch_device(ms, nn, rows, cols, low, ess, reps=1)
-    assert len(rows_) == 1 + 15 + 4 + 3
+    assert len(rows_) == 1 + 16 + 4 + 3
     assert all(r["same"] == 1 for r in rows_)
     assert min(r["wall_ms"] for r in rows_) > 0
     st = openph_amd.pms_batch_device(ms, nn, rows, cols, low, ess, options=best)
