@@ -1218,7 +1218,7 @@ __device__ __forceinline__ unsigned long long team_alloc(const Team &t, const De
 struct Phase2Counters {
     unsigned long long nadd = 0, nel = 0;
     int spills = 0;
-    bool ovf = false;
+    int ovf = 0;
 };
 
 // The Phase II chain of column j (pms_functions.cu:87-112) by one team. The
@@ -1237,21 +1237,26 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
     unsigned coff = d.off[j];
     int clen = d.len[j], ccap = d.cap[j];
     int *sA = buf, *sB = buf + WA, *sC = buf + WA + W;
-    // contents in the pool instead of LDS; in LDS they end with the sentinel
-    // kIntMax, so a column there holds fewer than W entries
-    bool glob = clen >= W;
-    bool handoff = HANDOFF && glob;
-    if (!glob) {
+    // The column's state as bits of one int: a divergent bool carried through
+    // the loop would be a lane mask, merged at every join of the loop body.
+    // kGlob: the contents are in the pool instead of LDS (in LDS they end
+    // with the sentinel kIntMax, so a column there holds fewer than W
+    // entries); kHand: handed off; kDirty: changed; kPivot: becomes the pivot
+    // of its low; kLost: progress held only in LDS could not be written back;
+    // kStop: the chain ends here.
+    constexpr int kGlob = 1, kHand = 2, kDirty = 4, kPivot = 8, kLost = 16, kStop = 32;
+    int fl = clen >= W ? (HANDOFF ? kGlob | kHand : kGlob) : 0;
+    if (!(fl & kGlob)) {
         for (int i = gl; i < clen; i += G) sB[i] = d.pool[coff + i];
         sB[clen] = kIntMax;
     }
     t.sync();
-    bool dirty = false, becomes_pivot = false, lost = false;
-    while (!handoff && pd.x > -1 && pd.x < j) {
+    // one exit, at the top
+    while (!(fl & (kHand | kStop)) && pd.x > -1 && pd.x < j) {
         const int na = pd.z;
         const unsigned aoff = (unsigned)pd.y;
-        int nc, nl;
-        if (!glob && na + clen < W) {
+        int nc = 0, nl = 0;
+        if (!(fl & kGlob) && na + clen < W) {
             const bool ain = na < WA;
             if (ain) {
                 for (int i = gl; i < na; i += G) sA[i] = d.pool[aoff + i];
@@ -1270,8 +1275,7 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
             t.sync();
             nl = nc ? sB[nc - 1] : -1;
         } else if (HANDOFF) {
-            handoff = true;
-            break;
+            fl |= kHand;
         } else {
             if (gl == 0) d.ess[lj] = 0;
             // the result may not fit in LDS: merge straight into a fresh pool
@@ -1279,36 +1283,41 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
             // and from the pool afterwards
             const unsigned long long base = team_alloc(t, d, (unsigned)(na + clen));
             if (base == ~0ull) {
-                ctr.ovf = true;
-                lost = !glob && dirty;  // progress held only in LDS
-                break;
+                ctr.ovf = 1;
+                fl |= kStop | ((fl & (kGlob | kDirty)) == kDirty ? kLost : 0);
+            } else {
+                const int *cur = (fl & kGlob) ? d.pool + coff : sB;
+                nc = team_xor(t, d.pool + aoff, na, cur, clen, d.pool + base);
+                t.fence();
+                if (!(fl & kGlob)) ctr.spills++;
+                fl |= kGlob;
+                coff = (unsigned)base;
+                ccap = na + clen;
+                nl = nc ? d.pool[base + nc - 1] : -1;
             }
-            const int *cur = glob ? d.pool + coff : sB;
-            nc = team_xor(t, d.pool + aoff, na, cur, clen, d.pool + base);
-            t.fence();
-            if (!glob) ctr.spills++;
-            glob = true;
-            coff = (unsigned)base;
-            ccap = na + clen;
-            nl = nc ? d.pool[base + nc - 1] : -1;
         }
-        ctr.nadd++;
-        ctr.nel += (unsigned long long)(na + clen + nc);
-        clen = nc;
-        dirty = true;
-        if (nl >= lj) {
-            if (gl == 0) atomicOr(&d.ctrl->err, kErrState);
-            break;
-        }
-        lj = nl;
-        if (lj < 0) break;
-        pd = d.pivdesc[lj];  // .w = left[lj]: one load for the rule and the pivot
-        d.clear[lj] = 1;
-        if (pd.w == j) {
-            becomes_pivot = true;
-            break;
+        if (!(fl & (kHand | kStop))) {
+            ctr.nadd++;
+            ctr.nel += (unsigned long long)(na + clen + nc);
+            clen = nc;
+            fl |= kDirty;
+            if (nl >= lj) {
+                if (gl == 0) atomicOr(&d.ctrl->err, kErrState);
+                fl |= kStop;
+            } else {
+                lj = nl;
+                if (lj < 0) {
+                    fl |= kStop;
+                } else {
+                    pd = d.pivdesc[lj];  // .w = left[lj]: one load for the rule and the pivot
+                    d.clear[lj] = 1;
+                    if (pd.w == j) fl |= kPivot | kStop;
+                }
+            }
         }
     }
+    const bool glob = fl & kGlob, dirty = fl & kDirty, lost = fl & kLost;
+    const bool handoff = fl & kHand, becomes_pivot = fl & kPivot;
     bool committed = false;
     if (!lost && dirty) {
         bool ok = true;
@@ -1320,7 +1329,7 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
                 want = (want + 7u) & ~7u;
                 const unsigned long long base = team_alloc(t, d, want);
                 if (base == ~0ull) {
-                    ctr.ovf = true;
+                    ctr.ovf = 1;
                     ok = false;  // nothing of this column's progress is kept
                 } else {
                     coff = (unsigned)base;
