@@ -384,16 +384,18 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
     if (tracking && (rc = track_state(c, tr, 0, s, m, ess_true))) return rc;
     launch_slots(d, s);
     // Phase II lanes per column: short input columns (simplicial complexes
-    // of small dimension) take 8 lanes with 80-entry LDS buffers (30 KB per
-    // block, so five blocks share a CU's 160 KB), longer ones (cubical
-    // complexes) a full wave with 1024-entry buffers.
+    // of small dimension) take 8 lanes with 80-entry LDS buffers (a 16-entry
+    // pivot buffer: 22 KB per block, so seven blocks share a CU's 160 KB),
+    // longer ones (cubical complexes) a full wave with 1024-entry buffers.
     Shape sh{cfg.lanes, cfg.wlds, 0, 0};
     if (!sh.lanes) {
         sh.lanes = maxlen <= 4 ? 8 : 64;
         sh.wlds = maxlen <= 4 ? 80 : 1024;
     }
     if (sh.lanes > 1) {
-        sh.grid = phase2_blocks_per_cu(sh.lanes, sh.wlds) * c.num_cu;
+        // twice the resident blocks: a block whose columns are done makes
+        // room for one that has not started, which evens out the load
+        sh.grid = 2 * phase2_blocks_per_cu(sh.lanes, sh.wlds) * c.num_cu;
         sh.long_grid = c.num_cu;
     }
     // the schedule: per-iteration reads when the trackers or the iteration
