@@ -37,20 +37,49 @@ struct SumOp {
     __device__ T operator()(T a, T b) const { return a + b; }
 };
 
+// x of the lane that DPP control CTRL selects, identity where it selects
+// none or the row is masked off
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_or(int identity, int x) {
+    return __builtin_amdgcn_update_dpp(identity, x, CTRL, ROW_MASK, 0xF, false);
+}
+
+// Inclusive scan of an int over the wave in registers (all 64 lanes
+// active): row shifts by 1, 2, 4 and 8 inside each row of 16 lanes, then the
+// row totals carried by the row broadcasts (lane 15 into rows 1 and 3, lane
+// 31 into rows 2 and 3).
+template <typename Op>
+__device__ __forceinline__ int wave_inclusive_scan(int x, Op op, int identity) {
+    x = op(x, dpp_or<0x111, 0xF>(identity, x));  // row_shr:1
+    x = op(x, dpp_or<0x112, 0xF>(identity, x));  // row_shr:2
+    x = op(x, dpp_or<0x114, 0xF>(identity, x));  // row_shr:4
+    x = op(x, dpp_or<0x118, 0xF>(identity, x));  // row_shr:8
+    x = op(x, dpp_or<0x142, 0xA>(identity, x));  // row_bcast:15
+    x = op(x, dpp_or<0x143, 0xC>(identity, x));  // row_bcast:31
+    return x;
+}
+
 // Exclusive scan over the whole block (blockDim.x multiple of 64). `lds`
 // needs blockDim.x/64 entries. Returns the exclusive prefix; *total gets the
 // block aggregate. Contains two __syncthreads(): every thread must call it.
+// Ints scan the wave in registers; wider types through lane shuffles.
 template <typename T, typename Op>
 __device__ T block_exclusive_scan(T v, Op op, T identity, T *lds, T *total) {
     const int lane = lane_id(), wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    T incl = v;
+    T incl, excl;
+    if constexpr (sizeof(T) == sizeof(int) && T(-1) < T(0)) {
+        incl = wave_inclusive_scan(v, op, identity);
+        excl = dpp_or<0x138, 0xF>(identity, incl);  // wave_shr:1
+    } else {
+        incl = v;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        T t = __shfl_up(incl, d, 64);
-        if (lane >= d) incl = op(incl, t);
+        for (int d = 1; d < 64; d <<= 1) {
+            T t = __shfl_up(incl, d, 64);
+            if (lane >= d) incl = op(incl, t);
+        }
+        excl = __shfl_up(incl, 1, 64);
+        if (lane == 0) excl = identity;
     }
-    T excl = __shfl_up(incl, 1, 64);
-    if (lane == 0) excl = identity;
     if (lane == 63) lds[wid] = incl;
     __syncthreads();
     T pre = identity, tot = identity;
