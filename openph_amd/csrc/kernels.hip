@@ -1107,6 +1107,19 @@ struct WaveTeam {
             if constexpr (G > 2) incl += dpp_up<2>(incl, gl);
             if constexpr (G > 4) incl += dpp_up<4>(incl, gl);
             if constexpr (G > 8) incl += dpp_up<8>(incl, gl);
+        } else if constexpr (G == 64) {
+            // the group is the wave: the register scan, and the total from
+            // lane 63 into a scalar register
+            incl = wave_inclusive_scan(v, [](int p, int q) { return p + q; }, 0);
+            total = __builtin_amdgcn_readlane(incl, 63);
+            return incl - v;
+        } else if constexpr (G == 32) {
+            // the register scan without the broadcast across the halves
+            incl += dpp_or<0x111, 0xF>(0, incl);  // row_shr:1
+            incl += dpp_or<0x112, 0xF>(0, incl);  // row_shr:2
+            incl += dpp_or<0x114, 0xF>(0, incl);  // row_shr:4
+            incl += dpp_or<0x118, 0xF>(0, incl);  // row_shr:8
+            incl += dpp_or<0x142, 0xA>(0, incl);  // row_bcast:15 into rows 1 and 3
         } else {
 #pragma unroll
             for (int o = 1; o < G; o <<= 1) {
