@@ -719,46 +719,46 @@ __global__ void __launch_bounds__(kBlock) k_tile(Dev d) {
 // Convergence test (is_reduced, pms_functions.cu:7-27: no non-negative low
 // occurs twice, from k_tile's flag; the first iteration always runs, as in
 // pms.cu:39-44), the iteration cap, and the exclusive max-scan of the tile
-// maxima over tiles, per slot. One block; wave s scans slot s with no block
-// barrier: each lane takes a contiguous run of tiles, the wave scans the
-// runs' maxima, and each lane writes its run's prefixes.
+// maxima over tiles, per slot. Block s scans slot s: each of its 1024
+// threads takes a contiguous run of tiles (independent loads), the block
+// scans the runs' maxima and each thread writes its run's prefixes. Every
+// block takes the loop's exit decision from the same inputs; block 0
+// writes it.
 __global__ void __launch_bounds__(1024) k_tilescan(Dev d) {
     __shared__ int s_halt;
+    __shared__ int lds[1024 / 64];
     if (threadIdx.x == 0) {
-        Ctrl *c = d.ctrl;
+        const Ctrl *c = d.ctrl;
         int h = c->halt;
         if (!h) {
             if (c->err) h = kHaltError;
             else if (c->iter > 1 && !c->any_dup) h = kHaltConverged;
             else if (c->iter > c->max_iters) h = kHaltIters;
-            c->halt = h;
+            if (h && blockIdx.x == 0) d.ctrl->halt = h;
         }
         s_halt = h;
     }
     __syncthreads();
     if (s_halt) return;
     const int nslots = d.ctrl->nslots;
-    const int s = threadIdx.x >> 6, lane = lane_id();
+    const int s = blockIdx.x;
     if (s >= nslots || s >= kMaxSlots) return;
-    const int per = (d.ntiles + 63) / 64;
-    const int t0 = lane * per;
+    const int per = (d.ntiles + 1023) / 1024;
+    const int t0 = threadIdx.x * per < d.ntiles ? threadIdx.x * per : d.ntiles;
     const int t1 = t0 + per < d.ntiles ? t0 + per : d.ntiles;
+    const int *tm = d.tile_max + s;
     int run = -1;
+#pragma unroll 4
     for (int t = t0; t < t1; t++) {
-        const int v = d.tile_max[t * kMaxSlots + s];
+        const int v = tm[t * kMaxSlots];
         run = v > run ? v : run;
     }
-    int incl = run;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int x = __shfl_up(incl, o, 64);
-        if (lane >= o && x > incl) incl = x;
-    }
-    int pre = __shfl_up(incl, 1, 64);
-    if (lane == 0) pre = -1;
+    int tot;
+    int pre = block_exclusive_scan(run, MaxOp<int>(), -1, lds, &tot);
+#pragma unroll 4
     for (int t = t0; t < t1; t++) {
         d.tile_pre[t * kMaxSlots + s] = pre;
-        const int v = d.tile_max[t * kMaxSlots + s];
+        const int v = tm[t * kMaxSlots];
         pre = v > pre ? v : pre;
     }
 }
@@ -865,7 +865,7 @@ void launch_phase1_mark(const Dev &d, hipStream_t s) {
     if (d.ntiles <= kFuseTiles) {
         k_mark<true><<<d.ntiles, kBlock, 0, s>>>(d);
     } else {
-        k_tilescan<<<1, 1024, 0, s>>>(d);
+        k_tilescan<<<kMaxSlots, 1024, 0, s>>>(d);
         k_mark<false><<<d.ntiles, kBlock, 0, s>>>(d);
     }
 }
