@@ -1489,6 +1489,47 @@ __global__ void __launch_bounds__(kLongBlock) k_phase2_long(Dev d) {
     }
 }
 
+// Columns handed off by lane groups whose buffers hold fewer than kMidW
+// entries: one wave per column with kMidW-entry LDS buffers (4 waves, 48 KB
+// per block), merging in the pool beyond that. No block barriers, and many
+// more columns in flight than one block per column gives.
+constexpr int kMidW = 1024;
+
+__global__ void __launch_bounds__(kBlock) k_phase2_mid(Dev d) {
+    constexpr int NW = kBlock / 64;
+    __shared__ __align__(16) int lds[NW * 3 * kMidW];
+    __shared__ unsigned long long s_add[NW], s_el[NW];
+    __shared__ int s_sp[NW], s_ovf[NW];
+    const WaveTeam<64> t{lane_id(), 0};
+    const int wid = threadIdx.x >> 6;
+    if (d.ctrl->halt) return;
+    const int nlong = d.ctrl->nlong;
+    Phase2Counters ctr;
+    for (int w = blockIdx.x * NW + wid; w < nlong; w += gridDim.x * NW)
+        phase2_column<WaveTeam<64>, kMidW, kMidW, false>(t, d, d.longw[w],
+                                                         lds + wid * 3 * kMidW, ctr);
+    // the counters are the same on every lane of a wave (one column team)
+    if (lane_id() == 0) {
+        s_add[wid] = ctr.nadd;
+        s_el[wid] = ctr.nel;
+        s_sp[wid] = ctr.spills;
+        s_ovf[wid] = ctr.ovf;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, e = 0;
+        int sp = 0, ovf = 0;
+        for (int k = 0; k < NW; k++) {
+            a += s_add[k];
+            e += s_el[k];
+            sp += s_sp[k];
+            ovf |= s_ovf[k];
+        }
+        if (ovf) d.ctrl->overflow = 1;
+        if (a || sp) stripe_add(d.ctrl, a, e, sp);
+    }
+}
+
 // The lane-group shapes that are compiled: {lanes, LDS entries per buffer}.
 using Phase2Kernel = void (*)(Dev);
 
@@ -1547,7 +1588,12 @@ void launch_phase2(const Dev &d, int lanes, int wlds, int grid, int long_grid, h
         return;
     }
     k<<<grid, kBlock, 0, s>>>(d);
-    k_phase2_long<<<long_grid, kLongBlock, 0, s>>>(d);
+    // handed-off columns: wave teams when the groups' buffers are smaller
+    // than theirs (three 48 KB blocks per CU), else the block kernel
+    if (wlds < kMidW)
+        k_phase2_mid<<<3 * long_grid, kBlock, 0, s>>>(d);
+    else
+        k_phase2_long<<<long_grid, kLongBlock, 0, s>>>(d);
 }
 
 // ---------------------------------------------------------------------------
