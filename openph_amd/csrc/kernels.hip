@@ -1468,21 +1468,23 @@ __global__ void __launch_bounds__(kBlock) k_phase2_group(Dev d) {
     }
 }
 
-// Columns handed off by the lane groups: one block of kLongBlock lanes per
-// column, 4096-entry LDS buffers, merging straight in the pool beyond that.
+// Columns handed off by the lane groups of 1024-entry buffers: one block of
+// kLongTeam lanes per column, 4096-entry LDS buffers (48 KB: three blocks
+// per CU), merging straight in the pool beyond that.
 constexpr int kLongW = 4096;
 constexpr int kLongBlock = kLongBlockLanes;
+constexpr int kLongTeam = 256;
 
-__global__ void __launch_bounds__(kLongBlock) k_phase2_long(Dev d) {
+__global__ void __launch_bounds__(kLongTeam) k_phase2_long(Dev d) {
     __shared__ int lds[3 * kLongW];
-    __shared__ int s_scan[kLongBlock / 64];
+    __shared__ int s_scan[kLongTeam / 64];
     __shared__ unsigned long long s_word;
-    const BlockTeam<kLongBlock> t{s_scan, &s_word};
+    const BlockTeam<kLongTeam> t{s_scan, &s_word};
     if (d.ctrl->halt) return;
     const int nlong = d.ctrl->nlong;
     Phase2Counters ctr;
     for (int w = blockIdx.x; w < nlong; w += gridDim.x)
-        phase2_column<BlockTeam<kLongBlock>, kLongW, kLongW, false>(t, d, d.longw[w], lds, ctr);
+        phase2_column<BlockTeam<kLongTeam>, kLongW, kLongW, false>(t, d, d.longw[w], lds, ctr);
     if (threadIdx.x == 0) {
         if (ctr.ovf) d.ctrl->overflow = 1;
         stripe_add(d.ctrl, ctr.nadd, ctr.nel, ctr.spills);
@@ -1593,7 +1595,7 @@ void launch_phase2(const Dev &d, int lanes, int wlds, int grid, int long_grid, h
     if (wlds < kMidW)
         k_phase2_mid<<<3 * long_grid, kBlock, 0, s>>>(d);
     else
-        k_phase2_long<<<long_grid, kLongBlock, 0, s>>>(d);
+        k_phase2_long<<<3 * long_grid, kLongTeam, 0, s>>>(d);
 }
 
 // ---------------------------------------------------------------------------
