@@ -1087,6 +1087,37 @@ __device__ __forceinline__ int mp_walk_lds(const int *a, int na, const int *b, i
     return cnt;
 }
 
+// mp_walk_lds<true> with one element of lookahead on each list, for long
+// walks: each step issues the read the step after next may need, so the
+// reads' latency overlaps the steps instead of adding to each. a[na] and
+// b[nb] hold kIntMax; indices are clamped there.
+__device__ __forceinline__ int mp_walk_lds_ahead(const int *a, int na, const int *b, int nb,
+                                                 int d0, int d1, int *c) {
+    int i = mp_split(a, na, b, nb, d0), k = d0 - i;
+    int av = a[i], bv = b[k];
+    int an = a[i + 1 < na ? i + 1 : na], bn = b[k + 1 < nb ? k + 1 : nb];
+    int prev_a = a[i > 0 ? i - 1 : 0];
+    prev_a = i > 0 ? prev_a : -1;
+    int cnt = 0;
+    for (int d = d0; d < d1; d++) {
+        const bool ta = av <= bv;
+        const int v = ta ? av : bv;
+        const bool keep = ta ? bv != av : prev_a != bv;
+        prev_a = ta ? av : prev_a;
+        i += ta ? 1 : 0;
+        k += ta ? 0 : 1;
+        // the element after the new head of the list just consumed
+        const int nx = ta ? a[i + 1 < na ? i + 1 : na] : b[k + 1 < nb ? k + 1 : nb];
+        av = ta ? an : av;
+        an = ta ? nx : an;
+        bv = ta ? bv : bn;
+        bn = ta ? bn : nx;
+        c[cnt] = v;
+        cnt += keep ? 1 : 0;
+    }
+    return cnt;
+}
+
 // A team merges one column: G lanes of one wave (WaveTeam) or a whole block
 // (BlockTeam). It provides its lane index and size, a barrier, an exclusive
 // scan with total, a broadcast from its leader and pool allocation.
@@ -1223,8 +1254,10 @@ __device__ int team_xor_into_b(const Team &t, const int *a, int na, int *b, int 
     const int l = t.lane();
     const int d0 = l * K < n ? l * K : n;
     const int d1 = d0 + K < n ? d0 + K : n;
-    const int cnt = LDS_A ? mp_walk_lds<true>(a, na, b, nb, d0, d1, c + d0)
-                          : mp_walk<true>(a, na, b, nb, d0, d1, c + d0);
+    int cnt;
+    if (!LDS_A) cnt = mp_walk<true>(a, na, b, nb, d0, d1, c + d0);
+    else if constexpr (Team::size >= 32) cnt = mp_walk_lds_ahead(a, na, b, nb, d0, d1, c + d0);
+    else cnt = mp_walk_lds<true>(a, na, b, nb, d0, d1, c + d0);
     int total;
     const int ex = t.scan(cnt, total);
     t.sync();
