@@ -137,6 +137,16 @@ def test_cubical64_full_size():
     assert np.array_equal(ess, ref.ess)
     # contractible grid: a single essential class
     assert int(ess.sum()) == 1
+    # 8 lanes with 80-entry buffers: the handed-off columns go to one wave
+    # each with 1024-entry buffers, and those that outgrow them merge in the
+    # pool
+    opt = openph_amd.Options()
+    opt.lanes_per_column = 8
+    low, ess, st = openph_amd.reduce(b, "pms", opt)
+    assert st.handoffs > 0 and st.spills > 0
+    assert np.array_equal(low, ref.low)
+    assert np.array_equal(ess, ref.ess)
+    assert st.additions == ref.additions
 
 
 def test_device_entry_point_with_torch_buffers():
@@ -168,6 +178,23 @@ def test_small_pool_compacts_and_stays_exact():
     assert st.compactions > 0 and st.overflows > 0
     assert np.array_equal(low, f["low"])
     assert np.array_equal(ess, f["ess"])
+
+
+def test_small_pool_with_handoffs_compacts_and_stays_exact():
+    """Compactions of a small pool while handed-off columns are merged by
+    the wave teams of the long-column pass."""
+    say("small pool, handoffs")
+    b = C.cubical64(seed=40, side=40)
+    ref = oracle.pms(b, snapshot=True)
+    opt = openph_amd.Options()
+    opt.lanes_per_column = 8
+    opt.slot_elems = 1
+    opt.pool_elems = b.nnz + b.m + 4096
+    low, ess, st = openph_amd.reduce(b, "pms", opt)
+    assert st.handoffs > 0
+    assert st.compactions > 0
+    assert np.array_equal(low, ref.low)
+    assert np.array_equal(ess, ref.ess)
 
 
 @pytest.mark.parametrize("extra", [0, 1024, 65536])
