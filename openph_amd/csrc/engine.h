@@ -31,7 +31,7 @@ struct Ctrl {
     int marks;                     // columns marked by Phase I (cumulative)
     int nwork;                     // columns queued for Phase II
     int spills;                    // Phase II columns that left LDS (cumulative)
-    int nlong;                     // columns handed to the block kernel
+    int nlong;                     // columns handed to the long-column pass
     int handoffs;                  // the same, cumulative
     // iteration control on the device, so that several iterations can be
     // queued (or replayed from a graph) between two reads of this block
@@ -89,7 +89,7 @@ struct Dev {
     int *left = nullptr, *beta = nullptr, *first = nullptr, *clear = nullptr;
     int *newpiv = nullptr, *slot = nullptr, *len = nullptr, *cap = nullptr;
     int *work = nullptr;                     // Phase II worklist
-    int *longw = nullptr;                    // columns handed to the block kernel
+    int *longw = nullptr;                    // columns handed to the long-column pass
     int4 *pivdesc = nullptr;                 // per row: {pivot column, off, len, left}
     unsigned *off = nullptr, *off2 = nullptr;
     unsigned long long *scan_tmp = nullptr;  // tiled-scan partials
@@ -121,8 +121,10 @@ void launch_phase1_detect(const Dev &d, hipStream_t s);
 void launch_phase1_mark(const Dev &d, hipStream_t s);
 void launch_apply1(const Dev &d, hipStream_t s);
 // lanes: 1 = one lane per column over all columns; 4..64 = lane groups over
-// the worklist with wlds LDS entries per buffer, followed by the block kernel
-// for the columns they hand off. grid / long_grid = blocks of the two.
+// the worklist with wlds LDS entries per buffer, followed by the long-column
+// pass for the columns they hand off (wave teams when wlds < 1024, else
+// 256-lane teams). grid = blocks of the groups; long_grid = CUs for the
+// long-column pass (three of its blocks per CU).
 void launch_phase2(const Dev &d, int lanes, int wlds, int grid, int long_grid, hipStream_t s);
 // Batches: local 1-based COO of `count` matrices -> global block-diagonal
 // COO (err gets bit 1 for an index outside its matrix), and global 1-based

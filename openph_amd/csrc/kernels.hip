@@ -1301,8 +1301,8 @@ struct Phase2Counters {
 // pivot column of at most WA entries is staged into sA (a longer one is read
 // from the pool), the survivors of the sum go to sC and then back to sB.
 // With HANDOFF, a column whose next sum may not fit W is committed as it
-// stands and queued on the long list for the block kernel, which continues
-// the same chain; without it, the team merges straight in the pool.
+// stands and queued on the long list for the long-column pass, which
+// continues the same chain; without it, the team merges straight in the pool.
 template <class Team, int W, int WA, bool HANDOFF>
 __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phase2Counters &ctr) {
     const int gl = t.lane();
@@ -1426,8 +1426,8 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
         }
         committed = ok;
     }
-    // a handed-off column continues in the block kernel from the state just
-    // committed (or from its unchanged state)
+    // a handed-off column continues in the long-column pass from the state
+    // just committed (or from its unchanged state)
     if (HANDOFF && handoff && (committed || !dirty) && gl == 0) {
         const int k = atomicAdd(&d.ctrl->nlong, 1);
         d.longw[k] = j;
@@ -1624,7 +1624,7 @@ void launch_phase2(const Dev &d, int lanes, int wlds, int grid, int long_grid, h
     }
     k<<<grid, kBlock, 0, s>>>(d);
     // handed-off columns: wave teams when the groups' buffers are smaller
-    // than theirs (three 48 KB blocks per CU), else the block kernel
+    // than theirs, else 256-lane teams (three 48 KB blocks per CU each)
     if (wlds < kMidW)
         k_phase2_mid<<<3 * long_grid, kBlock, 0, s>>>(d);
     else
