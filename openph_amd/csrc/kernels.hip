@@ -358,19 +358,21 @@ static int init_cols_blocks(int m) {
 // block. A row occurs in many columns (an edge in every triangle on it), so
 // the minimum column per row is first taken in an LDS hash table and then
 // issued once per distinct row and block.
-__global__ void __launch_bounds__(kBlock) k_entries(Dev d, const int *__restrict__ rows1,
-                                                   const int *__restrict__ cols1) {
+constexpr int kEntriesBlock = kTile;  // one entry per thread
+
+__global__ void __launch_bounds__(kEntriesBlock) k_entries(Dev d, const int *__restrict__ rows1,
+                                                          const int *__restrict__ cols1) {
     constexpr int H = 2 * kTile;
     __shared__ int hkey[H];
     __shared__ int hval[H];
-    for (int i = threadIdx.x; i < H; i += kBlock) {
+    for (int i = threadIdx.x; i < H; i += kEntriesBlock) {
         hkey[i] = -1;
         hval[i] = kIntMax;
     }
     __syncthreads();
     int bad = 0;
-    for (int k = 0; k < kItems; k++) {
-        const int e = blockIdx.x * kTile + k * kBlock + threadIdx.x;  // coalesced
+    for (int k = 0; k < kTile / kEntriesBlock; k++) {
+        const int e = blockIdx.x * kTile + k * kEntriesBlock + threadIdx.x;  // coalesced
         if (e >= d.nnz) continue;
         const int c = cols1[e] - 1, r = rows1[e] - 1;
         int b = 0;
@@ -394,7 +396,7 @@ __global__ void __launch_bounds__(kBlock) k_entries(Dev d, const int *__restrict
     }
     if (bad) atomicOr(&d.ctrl->err, bad);
     __syncthreads();
-    for (int i = threadIdx.x; i < H; i += kBlock) {
+    for (int i = threadIdx.x; i < H; i += kEntriesBlock) {
         const int key = hkey[i];
         if (key > -1) atomicMin(&d.left[key], hval[i]);
     }
@@ -412,7 +414,7 @@ void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t 
     const int ge = (d.nnz + kTile - 1) / kTile;
     if (d.m) k_init_cols<<<init_cols_blocks(d.m), kBlock, 0, s>>>(d, rows1, cols1);
     launch_scan(d, 0, d.off, s);
-    if (d.nnz) k_entries<<<ge, kBlock, 0, s>>>(d, rows1, cols1);
+    if (d.nnz) k_entries<<<ge, kEntriesBlock, 0, s>>>(d, rows1, cols1);
 }
 
 // Pseudo-dimension slots: pseudo-dim = len - 1 (dimensions.cu:61-63). The
