@@ -184,12 +184,13 @@ def _i32(x):
     return np.ascontiguousarray(np.asarray(x).reshape(-1), dtype=np.int32)
 
 
-def ph(algstr, r, c, m, col_width=1, low_true=None, device: int = 0):
+def ph(algstr, r, c, m, col_width=1, low_true=None, device: int = 0, schedule: int = 0):
     """The mex gateway's contract (src/matlab/ph/ph.cu:11-92): 1-based COO in,
     eight outputs back. Errors raise OpenPHError instead of printing.
 
     low_true (1-based, 0 = none) feeds the err_* trackers; without it they
-    keep the gateway's initial -1."""
+    keep the gateway's initial -1. schedule selects how the host drives the
+    pms iterations (Options.schedule; 0 = automatic, as the drop-in does)."""
     rows = _i32(r)
     cols = _i32(c)
     m = int(m)
@@ -207,6 +208,7 @@ def ph(algstr, r, c, m, col_width=1, low_true=None, device: int = 0):
     opt = Options()
     opt.device = device
     opt.track = 1
+    opt.schedule = int(schedule)
     st = Stats()
     rc = lib().openph_v2(algstr.encode(), _iptr(rows), _iptr(cols), m, nnz,
                          _iptr(lt) if lt is not None else None, ctypes.byref(opt), _iptr(low),

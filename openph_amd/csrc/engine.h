@@ -17,6 +17,10 @@ constexpr int kLongBlockLanes = 1024;  // lanes of the block kernels (long colum
 // Device-side scalars shared by all kernels of one reduction. Lives in
 // device memory; the host reads it back with one small copy.
 constexpr int kStripes = 16;
+// Iterations whose trackers and clock stamps a queued or replayed chunk keeps
+// on the device (iteration i in entry i % kIterRing); a chunk holds at most
+// half as many iterations, so its entries never collide.
+constexpr int kIterRing = 128;
 
 struct Ctrl {
     unsigned long long pool_top;   // next free pool entry
@@ -97,6 +101,12 @@ struct Dev {
     int *tile_max = nullptr, *tile_pre = nullptr;
     int *pool = nullptr, *pool2 = nullptr;   // column pool (int32 rows)
     Ctrl *ctrl = nullptr;
+    // per-iteration trackers and iteration times of the queued / graph
+    // schedules, read once per chunk (null when not wanted)
+    const int *ltrue = nullptr;              // 0-based low_true
+    unsigned long long *trk_part = nullptr;  // k_track_part's per-block sums
+    unsigned long long *trk_ring = nullptr;  // [kIterRing][4] per iteration
+    unsigned long long *its = nullptr;       // [kIterRing][2] start / end clock
 };
 
 // ---- launchers (kernels.hip) ----
@@ -169,6 +179,16 @@ void launch_final_ess(const Dev &d, hipStream_t s);
 void launch_classes_final(const Dev &d, hipStream_t s);
 // The trackers of the current state into Ctrl::trk_* (0-based low_true).
 void launch_track(const Dev &d, const int *low_true0, hipStream_t s);
+// Queued / graph schedules, after k_apply2: the end clock of the iteration
+// (the first write after the host cleared the ring wins, so instances queued
+// past a halt leave it alone) into its[], and the trackers of the state
+// (d.ltrue) into trk_ring[iter % kIterRing]. Instances queued past a halt
+// recompute the same entry from the same state.
+void launch_iter_end_stamp(const Dev &d, hipStream_t s);
+void launch_track_iter(const Dev &d, hipStream_t s);
+// Zeroes the clock ring before a chunk.
+void launch_clear_stamps(const Dev &d, hipStream_t s);
+int track_part_blocks(int m);
 // Compaction, step 1: new offsets of the live columns in off2 and the live
 // total in ctrl->pool_top (and scan_tmp[ntiles]).
 void launch_compact_scan(const Dev &d, hipStream_t s);

@@ -104,9 +104,14 @@ struct Context {
     int last_m = -1, last_nnz = -1, last_iters = 0;
     unsigned long long *d_stamps = nullptr, *h_stamps = nullptr;  // graph timing
     int wall_khz = 0;                                            // device clock rate
-    // trackers: 0-based low_true on the device
+    // trackers: 0-based low_true on the device; per-iteration trackers and
+    // clock stamps of the queued / graph schedules (device rings and their
+    // pinned host mirrors)
     int *t_low = nullptr;
     size_t c_tlow = 0;
+    unsigned long long *t_part = nullptr, *t_ring = nullptr, *t_its = nullptr;
+    size_t c_tpart = 0, c_tring = 0, c_tits = 0;
+    unsigned long long *h_ring = nullptr, *h_its = nullptr;
     // serial algorithms: column order and per-column clock stamps
     int *s_order = nullptr;
     unsigned long long *s_stamps = nullptr;
@@ -132,10 +137,12 @@ struct Context {
                         d.newpiv, d.slot, d.len, d.cap, d.work, d.longw, d.pivdesc, d.off, d.off2,
                         d.scan_tmp, d.slotmap, d.tile_max, d.tile_pre, d.pool, d.pool2, d.ctrl,
                         h_rows, h_cols, h_low, h_ess, b_moff, b_noff, b_rows, b_cols, b_err,
-                        t_low, s_order, s_stamps, f_key, f_key2, f_val, f_val2, f_pos, f_nnz,
-                        f_off, f_dist, f_tmp};
+                        t_low, t_part, t_ring, t_its, s_order, s_stamps, f_key, f_key2, f_val,
+                        f_val2, f_pos, f_nnz, f_off, f_dist, f_tmp};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
+        if (h_ring) (void)hipHostFree(h_ring);
+        if (h_its) (void)hipHostFree(h_its);
         if (h_ctrl) (void)hipHostFree(h_ctrl);
         if (h_word) (void)hipHostFree(h_word);
         if (h_stamps) (void)hipHostFree(h_stamps);
@@ -197,6 +204,10 @@ int build_cubical(Context &c, const float *field, int s0, int s1, int s2, int *r
 // the whole device).
 int reserve(Context &c, size_t m, size_t nnz, size_t pool_elems);
 int set_device(int device);
+// The pms reduction of one matrix held on the device (1-based COO); low1/ess
+// are device outputs, tr (may be null) the drop-in's trackers.
+int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int *low1, int *ess,
+            hipStream_t s, const RunCfg &cfg, openph_stats *stats, const Track *tr);
 // Reduction of a batch of `count` matrices held on the device (local 1-based
 // COO); low1/ess are device outputs.
 int run_batch(Context &c, int count, const int *ms, const int *nnzs, const int *rows1,

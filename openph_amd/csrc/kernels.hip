@@ -141,8 +141,12 @@ __device__ __forceinline__ unsigned slot_size(int len, int slot0) {
 // ---------------------------------------------------------------------------
 // Tiled exclusive scan of per-column sizes (pool layout).
 //   mode 0: setup layout, size = max(len, slot0)
-//   mode 1: compaction, size = 0 for cleared columns (never read again),
-//           len for final pivots (never written again), max(len, slot0) else
+//   mode 1: compaction, size = len for published pivots (read, never written
+//           again), 0 for other cleared columns (never read again),
+//           max(len, slot0) else. Whether a column is a published pivot
+//           comes from arglow (work[j] = 1 + the row it is the pivot of,
+//           written by k_pivot_rows), not from classes: an algorithm may
+//           mark a pivot's column positive, and its data must survive.
 // Offsets go to `out`; the total to scan_tmp[ntiles] and ctrl->pool_top.
 // ---------------------------------------------------------------------------
 
@@ -150,9 +154,8 @@ __device__ __forceinline__ unsigned col_size(const Dev &d, int j, int mode) {
     if (j >= d.m) return 0u;
     int n = d.len[j];
     if (mode == 0) return slot_size(n, d.slot0);
-    int c = d.classes[j];
-    if (c == 1) return 0u;
-    if (c == -1) return (unsigned)n;
+    if (d.work[j]) return (unsigned)n;
+    if (d.classes[j] == 1) return 0u;
     return slot_size(n, d.slot0);
 }
 
@@ -643,7 +646,10 @@ __global__ void __launch_bounds__(kBlock) k_first(Dev d) {
     if (d.ctrl->halt) return;
     // the iteration starts here: none of this kernel's blocks reads the
     // flags reset below
-    if (blockIdx.x == 0 && threadIdx.x == 0) iter_begin(d.ctrl);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        iter_begin(d.ctrl);
+        if (d.its) d.its[2 * (d.ctrl->iter % kIterRing)] = (unsigned long long)wall_clock64();
+    }
     for (int i = threadIdx.x; i < kHash; i += kBlock) {
         hkey[i] = -1;
         hval[i] = kIntMax;
@@ -1323,6 +1329,10 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
     // kStop: the chain ends here.
     constexpr int kGlob = 1, kHand = 2, kDirty = 4, kPivot = 8, kLost = 16, kStop = 32;
     int fl = clen >= W ? (HANDOFF ? kGlob | kHand : kGlob) : 0;
+    // this column's additions, merged elements and spills, counted once its
+    // state is committed: work thrown away by a full pool is not progress
+    unsigned long long padd = 0, pel = 0;
+    int pspill = 0;
     if (!(fl & kGlob)) {
         for (int i = gl; i < clen; i += G) sB[i] = d.pool[coff + i];
         sB[clen] = kIntMax;
@@ -1366,7 +1376,7 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
                 const int *cur = (fl & kGlob) ? d.pool + coff : sB;
                 nc = team_xor(t, d.pool + aoff, na, cur, clen, d.pool + base);
                 t.fence();
-                if (!(fl & kGlob)) ctr.spills++;
+                if (!(fl & kGlob)) pspill++;
                 fl |= kGlob;
                 coff = (unsigned)base;
                 ccap = na + clen;
@@ -1374,8 +1384,8 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
             }
         }
         if (!(fl & (kHand | kStop))) {
-            ctr.nadd++;
-            ctr.nel += (unsigned long long)(na + clen + nc);
+            padd++;
+            pel += (unsigned long long)(na + clen + nc);
             clen = nc;
             fl |= kDirty;
             if (nl >= lj) {
@@ -1427,6 +1437,11 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
             }
         }
         committed = ok;
+    }
+    if (committed) {
+        ctr.nadd += padd;
+        ctr.nel += pel;
+        ctr.spills += pspill;
     }
     // a handed-off column continues in the long-column pass from the state
     // just committed (or from its unchanged state)
@@ -1671,6 +1686,9 @@ __global__ void __launch_bounds__(kLongBlock) k_serial(Dev d, const int *__restr
             int clen = d.len[j], ccap = d.cap[j];
             int *sA = lds, *sB = lds + W, *sC = lds + 2 * W;
             bool glob = clen > W, dirty = false, lost = false;
+            // counted once the column's state is committed (see phase2_column)
+            unsigned long long padd = 0, pel = 0;
+            int pspill = 0;
             if (!glob)
                 for (int i = gl; i < clen; i += kLongBlock) sB[i] = d.pool[coff + i];
             t.sync();
@@ -1698,14 +1716,14 @@ __global__ void __launch_bounds__(kLongBlock) k_serial(Dev d, const int *__restr
                     const int *cur = glob ? d.pool + coff : sB;
                     nc = team_xor(t, d.pool + aoff, na, cur, clen, d.pool + base);
                     t.fence();
-                    if (!glob) ctr.spills++;
+                    if (!glob) pspill++;
                     glob = true;
                     coff = (unsigned)base;
                     ccap = na + clen;
                     nl = nc ? d.pool[base + nc - 1] : -1;
                 }
-                ctr.nadd++;
-                ctr.nel += (unsigned long long)(na + clen + nc);
+                padd++;
+                pel += (unsigned long long)(na + clen + nc);
                 clen = nc;
                 dirty = true;
                 if (nl >= lj) {
@@ -1719,11 +1737,16 @@ __global__ void __launch_bounds__(kLongBlock) k_serial(Dev d, const int *__restr
             if (ctr.ovf) {
                 // keep what is in the pool; the column is taken up again
                 // after compaction from the state committed here
-                if (!lost && dirty && glob && gl == 0) {
-                    d.off[j] = coff;
-                    d.len[j] = clen;
-                    d.cap[j] = ccap;
-                    d.low[j] = lj;
+                if (!lost && dirty && glob) {
+                    if (gl == 0) {
+                        d.off[j] = coff;
+                        d.len[j] = clen;
+                        d.cap[j] = ccap;
+                        d.low[j] = lj;
+                    }
+                    ctr.nadd += padd;
+                    ctr.nel += pel;
+                    ctr.spills += pspill;
                 }
                 stop = true;
             } else if (dirty) {
@@ -1751,6 +1774,11 @@ __global__ void __launch_bounds__(kLongBlock) k_serial(Dev d, const int *__restr
                     d.cap[j] = ccap;
                     d.low[j] = lj;
                 }
+                if (ok) {
+                    ctr.nadd += padd;
+                    ctr.nel += pel;
+                    ctr.spills += pspill;
+                }
                 stop = !ok;
             }
         }
@@ -1765,7 +1793,10 @@ __global__ void __launch_bounds__(kLongBlock) k_serial(Dev d, const int *__restr
             if (lj > -1) {
                 publish_pivot(d, lj, j);
                 d.classes[j] = -1;
-                d.classes[lj] = 1;
+                // the low's column is positive (standard.cu:31), unless it
+                // is itself a pivot, which only a matrix that is not a
+                // boundary matrix allows: its class stays negative
+                if (d.classes[lj] != -1) d.classes[lj] = 1;
                 d.ess[j] = 0;
                 d.ess[lj] = 0;
                 if (twist) d.low[lj] = -1;
@@ -2176,10 +2207,145 @@ void launch_track(const Dev &d, const int *low_true0, hipStream_t s) {
     if (d.m) k_track<<<(d.m + kBlock - 1) / kBlock, kBlock, 0, s>>>(d, low_true0);
 }
 
+// The same four quantities per block, overwritten (not accumulated), so that
+// an instance queued past a halt, which sees the same state, writes the same
+// values; k_track_fold sums them into the iteration's ring entry.
+constexpr int kTrackItems = 4;  // columns per thread
+constexpr int kTrackTile = kBlock * kTrackItems;
+
+int track_part_blocks(int m) { return (m + kTrackTile - 1) / kTrackTile; }
+
+__global__ void __launch_bounds__(kBlock) k_track_part(Dev d) {
+    __shared__ unsigned long long s_l1[kBlock / 64];
+    __shared__ int s_inf[kBlock / 64], s_un[kBlock / 64], s_es[kBlock / 64];
+    unsigned long long l1 = 0;
+    int inf = 0, un = 0, es = 0;
+#pragma unroll
+    for (int k = 0; k < kTrackItems; k++) {
+        const int t = blockIdx.x * kTrackTile + k * kBlock + threadIdx.x;
+        if (t < d.m) {
+            const long long diff = (long long)d.low[t] - (long long)d.ltrue[t];
+            const int a = (int)(diff < 0 ? -diff : diff);
+            l1 += (unsigned long long)a;
+            inf = a > inf ? a : inf;
+            un += d.classes[t] == 0;
+            es += d.ess[t] == 1;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        l1 += __shfl_down(l1, o, 64);
+        const int x = __shfl_down(inf, o, 64);
+        inf = x > inf ? x : inf;
+        un += __shfl_down(un, o, 64);
+        es += __shfl_down(es, o, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        s_l1[w] = l1;
+        s_inf[w] = inf;
+        s_un[w] = un;
+        s_es[w] = es;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; k++) {
+            l1 += s_l1[k];
+            inf = s_inf[k] > inf ? s_inf[k] : inf;
+            un += s_un[k];
+            es += s_es[k];
+        }
+        unsigned long long *p = d.trk_part + 4 * (size_t)blockIdx.x;
+        p[0] = l1;
+        p[1] = (unsigned long long)inf;
+        p[2] = (unsigned long long)un;
+        p[3] = (unsigned long long)es;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_track_fold(Dev d, int nblocks) {
+    __shared__ unsigned long long s[4][1024 / 64];
+    unsigned long long l1 = 0, inf = 0, un = 0, es = 0;
+    for (int b = threadIdx.x; b < nblocks; b += 1024) {
+        const unsigned long long *p = d.trk_part + 4 * (size_t)b;
+        l1 += p[0];
+        inf = p[1] > inf ? p[1] : inf;
+        un += p[2];
+        es += p[3];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        l1 += __shfl_down(l1, o, 64);
+        const unsigned long long x = __shfl_down(inf, o, 64);
+        inf = x > inf ? x : inf;
+        un += __shfl_down(un, o, 64);
+        es += __shfl_down(es, o, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        s[0][w] = l1;
+        s[1][w] = inf;
+        s[2][w] = un;
+        s[3][w] = es;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 1024 / 64; k++) {
+            l1 += s[0][k];
+            inf = s[1][k] > inf ? s[1][k] : inf;
+            un += s[2][k];
+            es += s[3][k];
+        }
+        unsigned long long *q = d.trk_ring + 4 * (size_t)(d.ctrl->iter % kIterRing);
+        q[0] = l1;
+        q[1] = inf;
+        q[2] = un;
+        q[3] = es;
+    }
+}
+
+__global__ void k_iter_end_stamp(Dev d) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    unsigned long long *e = d.its + 2 * (d.ctrl->iter % kIterRing) + 1;
+    if (*e == 0) *e = (unsigned long long)wall_clock64();
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+void launch_iter_end_stamp(const Dev &d, hipStream_t s) { k_iter_end_stamp<<<1, 64, 0, s>>>(d); }
+
+void launch_track_iter(const Dev &d, hipStream_t s) {
+    if (!d.m) return;
+    const int nb = track_part_blocks(d.m);
+    k_track_part<<<nb, kBlock, 0, s>>>(d);
+    k_track_fold<<<1, 1024, 0, s>>>(d, nb);
+}
+
+void launch_clear_stamps(const Dev &d, hipStream_t s) {
+    k_zero_u64<<<(2 * kIterRing + kBlock - 1) / kBlock, kBlock, 0, s>>>(d.its, 2 * kIterRing);
+}
+
 // ---------------------------------------------------------------------------
 // Pool compaction: live columns are copied, in column order, to the other
 // pool; cleared columns are dropped (never reduced again, never a pivot).
 // ---------------------------------------------------------------------------
+
+// work[] (free between iterations: k_apply1 rebuilds the worklist) marks the
+// published pivots for the compaction: work[p] = 1 + i where arglow[i] = p.
+__global__ void k_pivot_rows_clear(Dev d) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < d.m) d.work[j] = 0;
+}
+
+__global__ void k_pivot_rows(Dev d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.m) return;
+    const int p = d.arglow[i];
+    if (p > -1 && p < d.m) d.work[p] = i + 1;
+}
 
 __global__ void k_copy_cols(Dev d) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2196,13 +2362,18 @@ __global__ void k_copy_cols(Dev d) {
     int *dst = d.pool2 + no;
     for (int i = 0; i < n; i++) dst[i] = src[i];
     d.cap[j] = (int)sz;
-    if (d.classes[j] == -1) {
-        int l = d.low[j];
-        if (l > -1 && d.arglow[l] == j) d.pivdesc[l] = make_int4(j, (int)no, n, d.left[l]);
-    }
+    const int r = d.work[j] - 1;  // the row whose pivot j is, or -1
+    if (r > -1) d.pivdesc[r] = make_int4(j, (int)no, n, d.left[r]);
 }
 
-void launch_compact_scan(const Dev &d, hipStream_t s) { launch_scan(d, 1, d.off2, s); }
+void launch_compact_scan(const Dev &d, hipStream_t s) {
+    const int gm = (d.m + kBlock - 1) / kBlock;
+    if (d.m) {
+        k_pivot_rows_clear<<<gm, kBlock, 0, s>>>(d);
+        k_pivot_rows<<<gm, kBlock, 0, s>>>(d);
+    }
+    launch_scan(d, 1, d.off2, s);
+}
 
 void launch_compact_copy(Dev &d, hipStream_t s) {
     int gm = (d.m + kBlock - 1) / kBlock;

@@ -192,6 +192,8 @@ static int enqueue_iteration(const Dev &d, const Shape &sh, hipStream_t s, hipEv
     if (stamps) launch_stamp(d, stamps, 1, kStampRing, s);
     if (e1) OPH_CHECK(hipEventRecord(e1, s));
     launch_apply2(d, s);  // ends the iteration: Ctrl::halt for the next
+    if (d.its) launch_iter_end_stamp(d, s);
+    if (d.ltrue) launch_track_iter(d, s);
     return OPENPH_OK;
 }
 
@@ -284,6 +286,20 @@ static int graph_for(Context &c, const Shape &sh, int iters, bool timed, GraphEn
     return OPENPH_OK;
 }
 
+// Tracker slot `it` (tracker.cu:92-134) from the four exact integers of one
+// state: sum |low - low_true|, max |low - low_true|, #unreduced, #essential.
+static void fill_track(const Track *tr, int it, int m, long long ess_true, unsigned long long l1,
+                       unsigned long long linf, unsigned long long unreduced,
+                       unsigned long long ess) {
+    if (!tr || !tr->low_true1 || it < 0 || it > m) return;
+    if (tr->err_lone) tr->err_lone[it] = (float)l1;
+    if (tr->err_linf) tr->err_linf[it] = (float)linf;
+    if (tr->err_redu) tr->err_redu[it] = (float)unreduced / (float)m;
+    if (tr->err_ess)
+        tr->err_ess[it] =
+            (long long)ess == ess_true ? 1.0f : (float)ess_true / (float)(long long)ess;
+}
+
 // Trackers of the current state into slot `it` (tracker.cu:92-134); reads
 // the control block in any case.
 static int track_state(Context &c, const Track *tr, int it, hipStream_t s, int m,
@@ -293,16 +309,13 @@ static int track_state(Context &c, const Track *tr, int it, hipStream_t s, int m
     int rc = read_ctrl(c, s);
     if (rc || !slot) return rc;
     const Ctrl &h = *c.h_ctrl;
-    if (tr->err_lone) tr->err_lone[it] = (float)h.trk_l1;
-    if (tr->err_linf) tr->err_linf[it] = (float)h.trk_linf;
-    if (tr->err_redu) tr->err_redu[it] = (float)h.trk_unreduced / (float)m;
-    if (tr->err_ess)
-        tr->err_ess[it] = h.trk_ess == ess_true ? 1.0f : (float)ess_true / (float)h.trk_ess;
+    fill_track(tr, it, m, ess_true, h.trk_l1, (unsigned long long)h.trk_linf,
+               (unsigned long long)h.trk_unreduced, (unsigned long long)h.trk_ess);
     return OPENPH_OK;
 }
 
 // The pms driver (pms.cu:1-118) on device-resident 1-based COO.
-static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int *low1,
+int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int *low1,
                    int *ess, hipStream_t s, const RunCfg &cfg, openph_stats *stats,
                    const Track *tr) {
     if (m < 0 || nnz < 0) return OPENPH_EINVAL;
@@ -332,7 +345,8 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
         OPH_CHECK(hipGetDeviceProperties(&prop, c.device));
         c.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     }
-    if (!c.ev0) {
+    const bool events = true;
+    if (events && !c.ev0) {
         OPH_CHECK(hipEventCreate(&c.ev0));
         OPH_CHECK(hipEventCreate(&c.ev1));
     }
@@ -381,7 +395,45 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
     if ((rc = err_from_bits(c.h_ctrl->err))) return rc;
     lcap = c.h_ctrl->pool_cap;
     const int maxlen = c.h_ctrl->maxlen;
-    if (tracking && (rc = track_state(c, tr, 0, s, m, ess_true))) return rc;
+    // the schedule: graph replay when the last reduction on this workspace
+    // had the same size (its graphs are likely cached: a capture costs more
+    // than it saves once), queued chunks otherwise. The trackers and the
+    // iteration times (time_track) of the queued and graph schedules are
+    // kept per iteration on the device and read once per chunk; the sync
+    // schedule reads them after every iteration.
+    int sched = cfg.schedule;
+    if (sched == kSchedAuto)
+        sched = (c.last_m == m && c.last_nnz == nnz) ? kSchedGraph : kSchedQueued;
+    const bool iter_times = tr && tr->time_track;
+    const bool ring = sched != kSchedSync && (tracking || iter_times);
+    d.ltrue = nullptr;
+    d.trk_part = d.trk_ring = d.its = nullptr;
+    if (ring) {
+        if (!c.h_ring) {
+            OPH_CHECK(hipHostMalloc((void **)&c.h_ring, 4 * kIterRing * sizeof(unsigned long long),
+                                    hipHostMallocDefault));
+            OPH_CHECK(hipHostMalloc((void **)&c.h_its, 2 * kIterRing * sizeof(unsigned long long),
+                                    hipHostMallocDefault));
+        }
+        if (!c.wall_khz)
+            OPH_CHECK(hipDeviceGetAttribute(&c.wall_khz, hipDeviceAttributeWallClockRate,
+                                            c.device));
+        if (tracking) {
+            OPH_CHECK(grow(&c.t_part, c.c_tpart, 4 * (size_t)track_part_blocks(m)));
+            OPH_CHECK(grow(&c.t_ring, c.c_tring, 4 * (size_t)kIterRing));
+            d.ltrue = c.t_low;
+            d.trk_part = c.t_part;
+            d.trk_ring = c.t_ring;
+        }
+        if (iter_times) {
+            OPH_CHECK(grow(&c.t_its, c.c_tits, 2 * (size_t)kIterRing));
+            d.its = c.t_its;
+        }
+    }
+    if (tracking) {
+        if (ring) launch_track_iter(d, s);  // the state before Phase 0 (iter 0)
+        else if ((rc = track_state(c, tr, 0, s, m, ess_true))) return rc;
+    }
     launch_slots(d, s);
     // Phase II lanes per column: short input columns (simplicial complexes
     // of small dimension) take 8 lanes with 80-entry LDS buffers (a 16-entry
@@ -398,25 +450,13 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
         sh.grid = 2 * phase2_blocks_per_cu(sh.lanes, sh.wlds) * c.num_cu;
         sh.long_grid = c.num_cu;
     }
-    // the schedule: per-iteration reads when the trackers or the iteration
-    // times are wanted; else graph replay when the last reduction on this
-    // workspace had the same size (its graphs are likely cached: a capture
-    // costs more than it saves once), queued chunks otherwise
-    int sched = cfg.schedule;
-    const bool per_iter = tracking || (tr && tr->time_track);
-    if (per_iter) {
-        sched = kSchedSync;
-    } else if (sched == kSchedAuto) {
-        sched = (c.last_m == m && c.last_nnz == nnz) ? kSchedGraph : kSchedQueued;
-    }
-    const bool timed = stats != nullptr;
+    const bool timed = stats != nullptr && events;
 
-    OPH_CHECK(hipEventRecord(c.ev0, s));
+    if (events) OPH_CHECK(hipEventRecord(c.ev0, s));
     launch_phase0(d, s);
     // iteration i is timed from evs[2i] to evs[2i+1] (tic/toc around the
     // loop body, pms.cu:46-103; the trackers are outside)
     std::vector<hipEvent_t> &evs = c.iter_events;
-    const bool iter_times = tr && tr->time_track;
     int ncompact = 0, reads = 0, p2_launches = 0;
     float p2_ms = 0.f;
     const Ctrl &h = *c.h_ctrl;
@@ -485,7 +525,8 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
         int k = (c.last_m == m && c.last_nnz == nnz && c.last_iters > 0) ? c.last_iters : 4;
         int iter0 = 0;
         while (true) {
-            k = std::min(k, 64);
+            k = std::min(k, kIterRing / 2);
+            if (d.its) launch_clear_stamps(d, s);
             GraphEntry *ge = nullptr;
             if (sched == kSchedGraph && graph_for(c, sh, k, timed, &ge) != OPENPH_OK) {
                 // capture or instantiation refused: queue the same launches
@@ -506,6 +547,13 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
                                                 nullptr)))
                         return rc;
             }
+            if (d.trk_ring)
+                OPH_CHECK(hipMemcpyAsync(c.h_ring, d.trk_ring,
+                                         4 * kIterRing * sizeof(unsigned long long),
+                                         hipMemcpyDeviceToHost, s));
+            if (d.its)
+                OPH_CHECK(hipMemcpyAsync(c.h_its, d.its, 2 * kIterRing * sizeof(unsigned long long),
+                                         hipMemcpyDeviceToHost, s));
             if ((rc = read_ctrl(c, s))) return rc;
             reads++;
             // iterations of this chunk that reached Phase II: all that began,
@@ -513,6 +561,24 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
             const int begun = h.iter - iter0;
             const int ran2 =
                 begun - ((h.halt == kHaltConverged || h.halt == kHaltIters) ? 1 : 0);
+            if (d.trk_ring && iter0 == 0) {
+                const unsigned long long *q = c.h_ring;  // before Phase 0
+                fill_track(tr, 0, m, ess_true, q[0], q[1], q[2], q[3]);
+            }
+            for (int q = 0; q < ran2 && q < k; q++) {
+                const int i = iter0 + q + 1;
+                if (i > m) break;
+                const int e = i % kIterRing;
+                if (d.trk_ring) {
+                    const unsigned long long *x = c.h_ring + 4 * e;
+                    fill_track(tr, i, m, ess_true, x[0], x[1], x[2], x[3]);
+                }
+                if (d.its) {
+                    const unsigned long long a = c.h_its[2 * e], b = c.h_its[2 * e + 1];
+                    tr->time_track[i] =
+                        c.wall_khz > 0 && b >= a ? (float)((double)(b - a) / c.wall_khz) : 0.f;
+                }
+            }
             if (timed && ge) {
                 for (int q = 0; q < ran2 && q < k; q++) {
                     const int i = iter0 + q + 1;  // the iteration's stamps
@@ -546,10 +612,18 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
     c.last_iters = num_iters;
     if (tracking) {
         // the state the reference tracks after its last iteration
-        // (pms.cu:94-100): ess_hat_final, update_classes_final
+        // (pms.cu:94-100): ess_hat_final, update_classes_final; its
+        // trackers arrive with the last read of the control block
         launch_final_ess(d, s);
         launch_classes_final(d, s);
-        if ((rc = track_state(c, tr, num_iters - 1, s, m, ess_true))) return rc;
+        launch_track(d, c.t_low, s);
+    }
+    launch_final(d, low1, ess, s);
+    if (events) OPH_CHECK(hipEventRecord(c.ev1, s));
+    if ((rc = read_ctrl(c, s))) return rc;
+    if (tracking) {
+        fill_track(tr, num_iters - 1, m, ess_true, h.trk_l1, (unsigned long long)h.trk_linf,
+                   (unsigned long long)h.trk_unreduced, (unsigned long long)h.trk_ess);
         // scaling by the norms of low_true, over all m+1 entries
         // (openph.cu:188-193)
         for (int i = 0; i <= m; i++) {
@@ -557,9 +631,6 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
             if (tr->err_linf) tr->err_linf[i] = tr->err_linf[i] / norminf;
         }
     }
-    launch_final(d, low1, ess, s);
-    OPH_CHECK(hipEventRecord(c.ev1, s));
-    if ((rc = read_ctrl(c, s))) return rc;
     if (stats) {
         stats->num_iters = num_iters;
         stats->compactions = ncompact;
@@ -568,7 +639,7 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
         stats->merge_elems = (int64_t)h.merge_elems;
         stats->pool_peak = (int64_t)std::max<unsigned long long>(h.pool_peak, (unsigned long long)nnz);
         float ms = 0.f;
-        OPH_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+        if (events) OPH_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
         stats->device_ms = ms;
         stats->m = m;
         stats->lanes = sh.lanes;
@@ -582,11 +653,12 @@ static int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nn
     }
     if (iter_times) {
         tr->time_track[0] = 0.f;
-        for (int i = 1; i < num_iters && i <= m; i++) {
-            float ms = 0.f;
-            OPH_CHECK(hipEventElapsedTime(&ms, evs[2 * i], evs[2 * i + 1]));
-            tr->time_track[i] = ms;
-        }
+        if (sched == kSchedSync)
+            for (int i = 1; i < num_iters && i <= m; i++) {
+                float ms = 0.f;
+                OPH_CHECK(hipEventElapsedTime(&ms, evs[2 * i], evs[2 * i + 1]));
+                tr->time_track[i] = ms;
+            }
     }
     return OPENPH_OK;
 }
@@ -702,12 +774,14 @@ static int run_serial(Context &c, const int *rows1, const int *cols1, int m, int
         if ((rc = compact(c, cfg, s, &ncompact, lcap))) return rc;
         launch_iter_begin(d, s);  // clears the overflow flag
     }
-    if (algo == kAlgoRowAll) {
-        launch_ess_at_lows(d, s);
+    // standard and twist keep the ess they set column by column, with no
+    // final pass (standard.cu:18-54, twist.cu:16-87): twist never takes the
+    // columns of pseudo-dimension 0 (twist.cu:30), whose lows stay as given
+    if (algo == kAlgoRowAll) launch_ess_at_lows(d, s);
+    if (algo == kAlgoRowAll || algo == kAlgoStandard || algo == kAlgoTwist)
         launch_output(d, low1, ess, s);
-    } else {
+    else
         launch_final(d, low1, ess, s);
-    }
     OPH_CHECK(hipEventRecord(c.ev1, s));
     if ((rc = read_ctrl(c, s))) return rc;
     if (stats) {

@@ -208,14 +208,18 @@ def _expected_trackers(trk, m, lt1, n):
     return linf / norminf, lone / norm1, redu, essr
 
 
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3])
 @pytest.mark.parametrize("name", ["vr12", "random1", "vr40", "torus100", "cubical16"])
-def test_trackers_match_oracle(name):
-    say("trackers", name)
+def test_trackers_match_oracle(name, schedule):
+    """Every schedule fills the same trackers: the sync one reads them after
+    each iteration, the queued and graph ones keep them per iteration on the
+    device and read them once per chunk (the default)."""
+    say("trackers", name, "schedule", schedule)
     f = np.load(os.path.join(GOLD, name + ".npz"))
     b = _gen(name)
     r, c = b.coo1()
     lt1 = (f["low"] + 1).astype(np.int32)
-    o = openph_amd.ph("pms", r, c, b.m, 8, lt1)
+    o = openph_amd.ph("pms", r, c, b.m, 8, lt1, schedule=schedule)
     low, ess, err_linf, err_lone, err_redu, err_ess, time_track, nit = o
     n = int(nit[0])
     ref = oracle.pms(b, snapshot=True, low_true=f["low"].astype(np.int32))
@@ -267,6 +271,46 @@ def test_dropin_void_openph_fills_trackers_like_ph():
                             ip(ess), fp(errs[0]), fp(errs[1]), fp(errs[2]), fp(errs[3]), fp(tt),
                             ip(nit), 0)
     assert nit[0] == -3
+
+
+def test_dropin_with_trackers_does_not_force_the_sync_schedule():
+    """The reference's callers always pass low_true (zeros,
+    numerics/benchmark_datasets.m:53): with it the drop-in still queues its
+    iterations (one control-block read per chunk) and its outputs and
+    trackers equal the sync schedule's."""
+    say("dropin schedule")
+    f = np.load(os.path.join(GOLD, "torus100.npz"))
+    b = _gen("torus100")
+    r, c = b.coo1()
+    m = b.m
+    lt = np.zeros(m, np.int32)
+    outs = {}
+    for sched in (1, 0, 0):
+        low = np.full(m, -1, np.int32)
+        ess = np.full(m, -1, np.int32)
+        trk = np.full((5, m + 1), -1.0, np.float32)
+        trk[4, :] = 0.0
+        opt = openph_amd.Options()
+        opt.track = 1
+        opt.schedule = sched
+        st = openph_amd.Stats()
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))  # noqa: E731
+        rc = openph_amd.lib().openph_v2(b"pms", ip(r), ip(c), m, r.shape[0], ip(lt),
+                                        ctypes.byref(opt), ip(low), ip(ess),
+                                        trk.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                        ctypes.byref(st))
+        assert rc == 0
+        assert np.array_equal(low, f["low"] + 1) and np.array_equal(ess, f["ess"])
+        n = st.num_iters
+        assert trk[4, 0] == 0 and (trk[4, 1:n] > 0).all() and (trk[4, n:] == 0).all()
+        if sched == 0:
+            assert st.schedule in (2, 3) and st.host_reads <= 3
+        outs.setdefault(sched, []).append((trk[:4].copy(), n, st.schedule))
+    ref, n1, _ = outs[1][0]
+    for t, n, _ in outs[0]:
+        assert n == n1 and np.array_equal(t, ref)
+    # the second call of the same size replays graphs
+    assert outs[0][1][2] == 3
 
 
 def test_low_true_out_of_range_is_rejected():

@@ -94,3 +94,26 @@ def test_serial_edge_cases(algo):
                    dim=np.zeros(0, np.int32), value=np.zeros(0))
     low, ess, st = openph_amd.reduce(b, algo)
     assert low.size == 0 and st.num_iters == 1
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("algo", ["standard", "twist"])
+def test_serial_small_pool_on_non_boundary_matrices(algo, seed):
+    """On a matrix that is not a boundary matrix, the column of a low can
+    itself be a pivot, which the serial algorithms then mark positive
+    (standard.cu:31) or clear (twist.cu:1-7). A compaction must still keep
+    that pivot's data: liveness comes from arglow, not from classes."""
+    say("serial non-boundary small pool", algo, seed)
+    b = C.random_upper(np.random.default_rng(3000 + seed), 1200)
+    opt = openph_amd.Options()
+    opt.slot_elems = 1
+    opt.pool_elems = int(np.maximum(np.diff(b.colptr), 1).sum())
+    low, ess, st = openph_amd.reduce(b, algo, opt)
+    assert st.compactions > 0
+    if algo == "standard":
+        dl, de = oracle.dense_reduce(b)
+        assert np.array_equal(low, dl) and np.array_equal(ess, de)
+    ref = oracle.twist(b) if algo == "twist" else oracle.standard(b)
+    assert np.array_equal(low, ref.low) and np.array_equal(ess, ref.ess)
+    # additions are counted once, when the column's state is kept
+    assert st.additions == ref.additions and st.merge_elems == ref.merge_elems
