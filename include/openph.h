@@ -36,6 +36,8 @@ extern "C" {
 #define OPENPH_EITER -6       /* iteration cap reached                       */
 #define OPENPH_EDIMS -7       /* more than OPENPH_MAX_DIMS pseudo-dimensions */
 #define OPENPH_EBUSY -8       /* served queue full / admission refused       */
+#define OPENPH_ESCRATCH -9    /* input beyond the static scratch of
+                                 openph_pms_device_noalloc                   */
 
 #define OPENPH_MAX_DIMS 16
 
@@ -162,6 +164,27 @@ int openph_v2(const char *algstr, const int *rows1, const int *cols1, int m, int
 int openph_pms_device(const int *rows1, const int *cols1, int m, int nnz,
                       int *low1, int *ess, void *stream, const openph_options *opt,
                       openph_stats *stats);
+
+/*
+ * openph_pms_device without any allocation: no counterpart in the reference
+ * (whose openph() allocates and frees every buffer per call,
+ * src/cuda/openph.cu:72-230). For callers that cannot allocate device memory:
+ * code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc. Arguments as openph_pms_device (device pointers; the caller owns
+ * inputs and outputs). Every working buffer lives in one static device array
+ * of about 81 MB that each GPU holds from the library's load until the
+ * process ends. Limits: m <= 196,608 columns and nnz <= 589,824 entries
+ * (OPENPH_ESCRATCH beyond them, before any work), and column pools of
+ * 8,388,608 entries (OPENPH_ENOMEM when the initial layout, nnz +
+ * slot_elems per column, or the live columns do not fit). The iterations are
+ * always queued (schedule 1 or 2; a request for graphs queues them), the
+ * trackers are not available, and stats carry counters but no times. Calls on
+ * one device share the array: they are serialised by a lock, and each returns
+ * once low1/ess are written; the array keeps the last call's contents.
+ */
+int openph_pms_device_noalloc(const int *rows1, const int *cols1, int m, int nnz, int *low1,
+                              int *ess, void *stream, const openph_options *opt,
+                              openph_stats *stats);
 
 /*
  * Batched reduction of `count` independent matrices in one pass of the

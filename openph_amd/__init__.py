@@ -135,6 +135,8 @@ def lib():
                                         ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.POINTER(Options),
                                         ctypes.POINTER(Stats)]
+        L.openph_pms_device_noalloc.restype = ctypes.c_int
+        L.openph_pms_device_noalloc.argtypes = L.openph_pms_device.argtypes
         L.openph_pms_batch.restype = ctypes.c_int
         L.openph_pms_batch.argtypes = [ctypes.c_int, c_int_p, c_int_p, c_int_p, c_int_p,
                                        ctypes.POINTER(Options), c_int_p, c_int_p,

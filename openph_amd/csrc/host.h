@@ -188,6 +188,7 @@ struct RunCfg {
     int wlds = 0;
     bool track = false;
     bool timing = true;
+    bool no_alloc = false;  // the context's buffers only: see run_pms
 };
 
 RunCfg cfg_from(const openph_options *opt, int nnz);

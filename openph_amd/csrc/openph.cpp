@@ -314,7 +314,10 @@ static int track_state(Context &c, const Track *tr, int it, hipStream_t s, int m
     return OPENPH_OK;
 }
 
-// The pms driver (pms.cu:1-118) on device-resident 1-based COO.
+// The pms driver (pms.cu:1-118) on device-resident 1-based COO. With
+// cfg.no_alloc it allocates nothing: the caller's context holds every buffer
+// (ensure() then only checks sizes) and host mirrors, no events are created
+// (stats carry no times) and the iterations are queued, never captured.
 int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int *low1,
                    int *ess, hipStream_t s, const RunCfg &cfg, openph_stats *stats,
                    const Track *tr) {
@@ -345,7 +348,9 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
         OPH_CHECK(hipGetDeviceProperties(&prop, c.device));
         c.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     }
-    const bool events = true;
+    const bool events = !cfg.no_alloc;
+    if (cfg.no_alloc && (!c.h_ctrl || !c.h_word || (tr && (tr->low_true1 || tr->time_track))))
+        return OPENPH_EINVAL;
     if (events && !c.ev0) {
         OPH_CHECK(hipEventCreate(&c.ev0));
         OPH_CHECK(hipEventCreate(&c.ev1));
@@ -404,6 +409,7 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
     int sched = cfg.schedule;
     if (sched == kSchedAuto)
         sched = (c.last_m == m && c.last_nnz == nnz) ? kSchedGraph : kSchedQueued;
+    if (cfg.no_alloc && sched == kSchedGraph) sched = kSchedQueued;
     const bool iter_times = tr && tr->time_track;
     const bool ring = sched != kSchedSync && (tracking || iter_times);
     d.ltrue = nullptr;
@@ -974,6 +980,8 @@ const char *openph_strerror(int status) {
         case OPENPH_EITER: return "iteration cap reached";
         case OPENPH_EDIMS: return "too many distinct pseudo-dimensions";
         case OPENPH_EBUSY: return "admission refused";
+        case OPENPH_ESCRATCH:
+            return "input larger than the static scratch of the no-allocation entry point";
         default: return "unknown status";
     }
 }

@@ -35,7 +35,7 @@ def test_library_loads_and_exports_header_symbols():
 def test_version_and_strerror():
     L = openph_amd.lib()
     assert L.openph_version().startswith(b"openph-mi355x")
-    for code in range(0, -9, -1):
+    for code in range(0, -10, -1):
         assert L.openph_strerror(code)
 
 
