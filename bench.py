@@ -14,10 +14,17 @@ Also reported, on the same JSON line:
 * hbm_alg: B_alg = B_merge + B_io + B_state of the whole reduction
   (B_io = 8 nnz + 8 m, B_state = 36 m * iterations) over device time.
 * cpu_baseline: the reference's serial twist on one core (its own
-  reduce_col_host, oracle/_ref/libref.so, when that build is present; else
-  the oracle's fixed-slot restatement) on the same matrix.
+  reduce_col_host, oracle/_ref/libref.so, when that build is present and its
+  fixed-slot layout fits; else the oracle's restatement on growable columns)
+  on the same matrix, with the host's CPU model and core counts.
+* dropin: the C symbol openph() host to host with an all-zeros low_true, as
+  the reference's callers drive it (trackers filled every iteration).
+* cubical64: the same figures (rate, Phase II roofline, CPU twist) on the
+  cubical lower-star filtration of a 64^3 grid, the largest single matrix.
+* batched / served: the 512-filtration batch and the request streams against
+  the plain drop-in; the batched leg carries its own Phase II roofline.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs a,b,...] [--no-cpu]
 """
 import argparse
 import json
@@ -34,45 +41,233 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 
 
-def cpu_baseline(b, budget_s: float = 10.0):
+def host_cpu():
+    """CPU model and core counts of the host the baseline runs on."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cores": usable}
+
+
+def _timed_runs(fn, budget_s, min_runs=3, max_runs=200):
+    times, out = [], None
+    t_end = time.perf_counter() + budget_s
+    while time.perf_counter() < t_end or len(times) < min_runs:
+        t0 = time.perf_counter()
+        out = fn()
+        times.append(time.perf_counter() - t0)
+        if len(times) >= max_runs:
+            break
+    return times, out
+
+
+def cpu_baseline(b, name: str, budget_s: float = 10.0):
+    """The reference's serial twist on one host core, on the whole matrix.
+
+    With the reference's own reduce_col_host / left_to_right_host compiled
+    from its sources (oracle/_ref/libref.so) when that build is present and
+    its fixed-slot layout (2 m p ints, p = col_width * longest input column)
+    stays under 4 GiB; otherwise the oracle's C restatement of the same twist
+    on growable columns. Each run times the column layout and the reduction
+    (the reference's create_rows_mp and twist loop), not its per-column
+    tracker or event calls."""
     import oracle
 
     oracle.build()
-    kind = "port"
-    ref_ok = False
     try:
         ref_ok = oracle.ref_available()
     except OSError:
         ref_ok = False
-    # slots per column for the fixed-slot layout: the longest column the
-    # reduction produces, over the longest input column, plus one
-    probe = oracle.twist(b)
+    probe = oracle.twist_dyn(b)
     mx = max(1, int(np.diff(b.colptr).max()))
+    # slots per column: the longest column the reduction produces, plus one
     cw = -(-max(probe.max_len, mx) // mx) + 1
-    times = []
-    t_end = time.perf_counter() + budget_s
-    while time.perf_counter() < t_end or len(times) < 3:
-        t0 = time.perf_counter()
-        if ref_ok:
-            low, ess = oracle.ref_run("twist", b, cw)
-            kind = "reference"
-        else:
-            low = oracle.twist(b, col_width=cw).low
-        times.append(time.perf_counter() - t0)
-        if len(times) >= 200:
-            break
+    slot_bytes = 2 * 4 * (b.m + 1) * cw * mx
+    use_ref = ref_ok and slot_bytes <= (4 << 30)
+    if use_ref:
+        times, res = _timed_runs(lambda: oracle.ref_run("twist", b, cw), budget_s)
+        low = res[0]
+        kind, how = "reference", (f"the reference's reduce_col_host compiled from its sources, "
+                                  f"fixed slots p={cw * mx}")
+    else:
+        times, res = _timed_runs(lambda: oracle.twist_dyn(b), budget_s)
+        low = res.low
+        kind = "port"
+        how = ("the oracle's C restatement of the reference's twist on growable columns"
+               + (f" (the reference's fixed-slot layout, p={cw * mx}, would take "
+                  f"{slot_bytes / 2**30:.1f} GiB)" if ref_ok else ""))
     med = statistics.median(times)
-    return {
+    out = {
         "value": b.m / med,
         "unit": "columns/s",
         "cores": 1,
         "kind": kind,
-        "sample": (f"serial twist (slots p={cw * mx}) on the full Torus-100 matrix, "
-                   f"{len(times)} runs, median {med * 1e3:.1f} ms; "
-                   + ("the reference's reduce_col_host compiled from its sources"
-                      if kind == "reference" else "C restatement of the reference's twist")),
+        "sample": (f"serial twist on the full {name} matrix (m={b.m}), {len(times)} runs of "
+                   f"about {budget_s:.0f} s, median {med * 1e3:.1f} ms; {how}"),
         "low_sha_match": None,
-    }, low
+    }
+    out.update(host_cpu())
+    return out, low
+
+
+def device_leg(b, name, steps, warmup, gold=None):
+    """`steps` full pms reductions of `b` through openph_pms_device, COO in
+    HBM before timing starts, low/ess left in HBM. Queued schedule, so that
+    the library brackets every Phase II launch with HIP events on the stream
+    it runs on. Returns the wall time, per-step stats and the result."""
+    import torch
+
+    import openph_amd
+
+    r, c = b.coo1()
+    dev = torch.device("cuda:0")
+    dr = torch.from_numpy(r).to(dev)
+    dc = torch.from_numpy(c).to(dev)
+    low = torch.empty(b.m, dtype=torch.int32, device=dev)
+    ess = torch.empty(b.m, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    opt = openph_amd.Options()
+    opt.schedule = openph_amd.Options.QUEUED
+    print(f"bench: {name} m={b.m} nnz={b.nnz}, warmup {warmup}, steps {steps}",
+          file=sys.stderr, flush=True)
+    for _ in range(warmup):
+        openph_amd.pms_device(dr, dc, b.m, low, ess, stream=stream, options=opt)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(openph_amd.pms_device(dr, dc, b.m, low, ess, stream=stream, options=opt))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    print(f"bench: {name}: {steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
+    ok = None
+    if gold is not None:
+        ok = bool(np.array_equal(low.cpu().numpy(), gold[0] + 1)
+                  and np.array_equal(ess.cpu().numpy(), gold[1]))
+    return wall, stats, ok
+
+
+def phase2_roofline(stats, m, kernel):
+    """Roofline of the Phase II launches: algorithmic bytes per launch =
+    (4 * sum over additions of (|a| + |b| + |a xor b|) + 12 * m * launches)
+    / launches (each launch reads low, arglow and classes of every column
+    once), over the average launch time (HIP events, median over steps)."""
+    st = stats[-1]
+    launches = max(1, st.phase2_launches)
+    p2_bytes = 4.0 * st.merge_elems + 12.0 * m * launches
+    p2_ms = statistics.median([s.phase2_ms for s in stats])
+    achieved = p2_bytes / (p2_ms * 1e-3) / 1e9 if p2_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": kernel,
+            "bytes_per_launch": p2_bytes / launches, "avg_launch_ms": p2_ms / launches,
+            "launches": st.phase2_launches}
+
+
+def b_alg(st, m, nnz):
+    """B_merge + B_io + B_state of one reduction (36 m bytes per iteration)."""
+    return 4.0 * st.merge_elems + 8.0 * nnz + 8.0 * m + 36.0 * m * (st.num_iters - 1)
+
+
+def cubical_leg(steps: int = 8, warmup: int = 2, cpu: bool = True):
+    """Plain pms on the cubical lower-star filtration of a 64^3 Gaussian field
+    (sigma 2, seed 0; m = 2,048,383), the largest single-matrix
+    configuration: column reductions/s, its Phase II roofline and the CPU
+    twist on the same matrix."""
+    from openph_amd import complexes as C
+
+    t0 = time.perf_counter()
+    b = C.cubical_lower_star(C.smoothed_field(np.random.default_rng(0), 64))
+    gen_s = time.perf_counter() - t0
+    wall, stats, _ = device_leg(b, "cubical64", steps, warmup)
+    st = stats[-1]
+    dev_ms = statistics.median([s.device_ms for s in stats])
+    value = b.m * steps / wall
+    roof = phase2_roofline(stats, b.m, f"Phase II: k_phase2_group<{st.lanes}, {st.group_lds}> "
+                                       "+ k_phase2_long")
+    # HBM bytes per Phase II pass from the committed counter runs
+    tfile = os.path.join(ROOT, "profiles", "pms_cubical64_traffic.json")
+    if os.path.exists(tfile):
+        tj = json.load(open(tfile))
+        if f"k_phase2_group<{st.lanes}, {st.group_lds}>" in tj.get("kernel", ""):
+            roof["traffic"] = tj["traffic_bytes_per_launch"]
+    out = {
+        "workload": "pms_cubical64_lower_star", "m": b.m, "nnz": b.nnz,
+        "field": "64^3 N(0,1) float32 seed 0, Gaussian smoothing sigma 2",
+        "cols_per_s": value, "ms_per_step": wall / steps * 1e3, "steps": steps,
+        "roofline": roof,
+        "hbm_alg": {"bytes": b_alg(st, b.m, b.nnz), "device_ms": dev_ms,
+                    "GBps": b_alg(st, b.m, b.nnz) / (dev_ms * 1e-3) / 1e9},
+        "pms": {"num_iters": st.num_iters, "additions": st.additions,
+                "merge_elems": st.merge_elems, "phase2_ms": statistics.median(
+                    [s.phase2_ms for s in stats]), "handoffs": st.handoffs,
+                "compactions": st.compactions, "pool_peak": st.pool_peak},
+        "host_generation_s": gen_s,
+    }
+    if cpu:
+        print("bench: cubical64 cpu baseline", file=sys.stderr, flush=True)
+        cb, cpu_low = cpu_baseline(b, "cubical 64^3")
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = value / cb["value"]
+        out["cpu_equals_gpu"] = None  # filled by main from the GPU result
+        out["_cpu_low"] = cpu_low
+        out["_b"] = b
+    return out
+
+
+def dropin_leg(b, reps: int = 20):
+    """The drop-in as the reference's callers drive it: the C symbol
+    openph() (src/cuda/openph.cu:29-35) host to host on Torus-100 with an
+    all-zeros low_true (numerics/benchmark_datasets.m:53), so the trackers
+    are filled every iteration, and the outputs initialised as the mex
+    gateway does (src/matlab/ph/ph.cu:72-78)."""
+    import ctypes
+
+    import openph_amd
+
+    L = openph_amd.lib()
+    r, c = b.coo1()
+    m = b.m
+    lt = np.zeros(m, np.int32)
+    low = np.full(m, -1, np.int32)
+    ess = np.full(m, -1, np.int32)
+    errs = [np.full(m + 1, -1, np.float32) for _ in range(4)]
+    tt = np.zeros(m + 1, np.float32)
+    nit = np.zeros(1, np.int32)
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))  # noqa: E731
+    fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))  # noqa: E731
+
+    def call():
+        L.openph(b"pms", ip(r), ip(c), m, 8, ip(lt), r.shape[0], ip(low), ip(ess), fp(errs[0]),
+                 fp(errs[1]), fp(errs[2]), fp(errs[3]), fp(tt), ip(nit), 0)
+
+    for _ in range(3):
+        call()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        call()
+        ts.append(time.perf_counter() - t0)
+    # the schedule the drop-in picks with trackers on (openph_v2, same options)
+    opt = openph_amd.Options()
+    opt.track = 1
+    st = openph_amd.Stats()
+    trk = np.zeros((5, m + 1), np.float32)
+    L.openph_v2(b"pms", ip(r), ip(c), m, r.shape[0], ip(lt), ctypes.byref(opt), ip(low), ip(ess),
+                fp(trk), ctypes.byref(st))
+    med = statistics.median(ts)
+    return {"workload": "openph('pms') on Torus-100, host COO in, host outputs, low_true = 0",
+            "ms_per_call": med * 1e3, "cols_per_s": m / med, "calls": reps,
+            "num_iters": int(nit[0]), "schedule": st.schedule, "host_reads": st.host_reads,
+            "time_track_sum_ms": float(tt[1:int(nit[0])].sum())}
 
 
 def batched_leg(count: int = 512, reps: int = 3, tune: bool = True):
@@ -147,6 +342,11 @@ def batched_leg(count: int = 512, reps: int = 3, tune: bool = True):
                   if tune else None),
         "batched_num_iters": st_b.num_iters, "batched_additions": st_b.additions,
         "batched_merge_elems": st_b.merge_elems,
+        "roofline": dict(phase2_roofline([st_b], M, f"k_phase2_group<{st_b.lanes}, "
+                                                     f"{st_b.group_lds}> + k_phase2_mid"),
+                         timing=("device-clock stamps around each Phase II launch "
+                                 "(graph schedule)" if st_b.schedule == 3 else
+                                 "HIP events around each Phase II launch")),
         "batched_equals_plain": same, "host_generation_s": gen_s,
     }
 
@@ -400,11 +600,56 @@ def _served_run(pool, coo, ref, seq, cols, qps, workers, rng):
     }
 
 
+LEGS = ("torus", "dropin", "cubical", "twist", "builders", "batched", "served", "cpu",
+        "batched_only")
+DEFAULT_LEGS = ("torus", "dropin", "cubical", "twist", "builders", "batched", "served", "cpu")
+
+
+def batched_only_leg(count: int = 512, reps: int = 5):
+    """The batched call alone (default options, no tuning and no plain
+    pass): the workload of the batched kernel profile."""
+    import torch
+
+    import openph_amd
+    from openph_amd import complexes as C
+
+    mats = C.batched_mix(count, seed=0)
+    ms = np.array([b.m for b in mats], np.int32)
+    nnzs = np.array([b.nnz for b in mats], np.int32)
+    rr, cc = zip(*(b.coo1() for b in mats))
+    dev = torch.device("cuda:0")
+    rows = torch.from_numpy(np.concatenate(rr)).to(dev)
+    cols = torch.from_numpy(np.concatenate(cc)).to(dev)
+    M = int(ms.sum())
+    low = torch.empty(M, dtype=torch.int32, device=dev)
+    ess = torch.empty(M, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    opt = openph_amd.Options()
+    opt.schedule = openph_amd.Options.QUEUED
+    stats, ts = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        stats.append(openph_amd.pms_batch_device(ms, nnzs, rows, cols, low, ess, stream=stream,
+                                                 options=opt))
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    st = stats[-1]
+    return {"workload": f"{count} VR 2-skeleta, N~U{{20..80}}, seed 0, default options, queued",
+            "columns": M, "batched_ms": statistics.median(ts) * 1e3, "reps": reps,
+            "roofline": phase2_roofline(stats, M, f"k_phase2_group<{st.lanes}, {st.group_lds}>"
+                                                  " + k_phase2_mid"),
+            "num_iters": st.num_iters, "additions": st.additions,
+            "merge_elems": st.merge_elems, "handoffs": st.handoffs}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--legs", default="all",
+                    help="comma-separated subset of " + ",".join(LEGS) + " (default all); "
+                         "'torus' is the headline and 'cpu' its CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
@@ -415,7 +660,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="batched leg with the default setting instead of the autotuner's")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run one 256 MiB device copy, the reference kernel that scales "
+                    help="also run one 2 GiB device copy, the reference kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -423,6 +668,16 @@ def main():
                                    "supported (independent matrices would shard across "
                                    "GPUs with no communication)" % args.gpus}))
         sys.exit(2)
+    legs = set(DEFAULT_LEGS) if args.legs == "all" else set(args.legs.split(","))
+    unknown = legs - set(LEGS)
+    if unknown:
+        ap.error("unknown legs: " + ",".join(sorted(unknown)))
+    for flag, leg in (("no_cpu", "cpu"), ("no_batched", "batched"), ("no_served", "served"),
+                      ("no_builders", "builders"), ("no_twist", "twist")):
+        if getattr(args, flag):
+            legs.discard(leg)
+            if leg == "cpu":
+                legs.discard("cubical_cpu")
 
     import torch
 
@@ -430,111 +685,96 @@ def main():
     from openph_amd import complexes as C
 
     openph_amd.lib()
-    b = C.torus100()
-    r, c = b.coo1()
     dev = torch.device("cuda:0")
-    dr = torch.from_numpy(r).to(dev)
-    dc = torch.from_numpy(c).to(dev)
-    low = torch.empty(b.m, dtype=torch.int32, device=dev)
-    ess = torch.empty(b.m, dtype=torch.int32, device=dev)
-    stream = torch.cuda.current_stream(dev)
-    # queued chunks of iterations: Phase II is bracketed by HIP events on the
-    # stream it runs on (graph replay would time it with device-clock stamps)
-    opt = openph_amd.Options()
-    opt.schedule = openph_amd.Options.QUEUED
-    print(f"bench: torus100 m={b.m} nnz={b.nnz}, warmup {args.warmup}", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        openph_amd.pms_device(dr, dc, b.m, low, ess, stream=stream, options=opt)
-    torch.cuda.synchronize()
-    stats = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        stats.append(openph_amd.pms_device(dr, dc, b.m, low, ess, stream=stream, options=opt))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    print(f"bench: {args.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
+    b = C.torus100()
+    m, nnz = b.m, b.nnz
+    gold = np.load(os.path.join(ROOT, "tests", "golden", "torus100.npz"))
+    out = {}
+    if "torus" in legs:
+        wall, stats, ok = device_leg(b, "torus100", args.steps, args.warmup,
+                                     (gold["low"], gold["ess"]))
+        st = stats[-1]
+        dev_ms = statistics.median([s.device_ms for s in stats])
+        kname = (f"k_phase2_group<{st.lanes}, {st.group_lds}>" if st.lanes > 1
+                 else "k_phase2_lane")
+        roof = phase2_roofline(stats, m, kname)
+        # HBM bytes per launch of the same kernel from the committed rocprofv3
+        # counter runs (FETCH_SIZE, WRITE_SIZE scaled by a plain copy)
+        tfile = os.path.join(ROOT, "profiles", "pms_torus100_traffic.json")
+        if os.path.exists(tfile):
+            tj = json.load(open(tfile))
+            if kname in tj.get("kernel", ""):
+                roof["traffic"] = tj["traffic_bytes_per_launch"]
+        value = m * args.steps / wall
+        out.update({
+            "metric": "column reductions/s",
+            "value": value,
+            "unit": "columns/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": wall / args.steps * 1e3,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "int32",
+            "data": "synthetic",
+            "config": {"workload": "pms_torus100_vr2", "m": m, "nnz": nnz,
+                       "points": 100, "torus": "R=1 r=0.4 sigma=0.03 seed=0",
+                       "algo": "pms", "inputs": "1-based COO resident in HBM"},
+            "roofline": roof,
+            "hbm_alg": {"bytes": b_alg(st, m, nnz), "device_ms": dev_ms,
+                        "GBps": b_alg(st, m, nnz) / (dev_ms * 1e-3) / 1e9 if dev_ms > 0 else 0.0},
+            "pms": {"num_iters": st.num_iters, "additions": st.additions,
+                    "schedule": st.schedule, "host_reads": st.host_reads,
+                    "merge_elems": st.merge_elems, "compactions": st.compactions,
+                    "pool_peak": st.pool_peak, "additions_per_s": st.additions * args.steps / wall,
+                    "parity_vs_golden": ok},
+        })
 
     if args.calibrate_copy:
-        src = torch.ones(64 << 20, dtype=torch.int32, device=dev)  # 256 MiB
-        # 1 GiB of other writes push src out of the 256 MiB last-level cache,
-        # so the copy below reads it from HBM
-        evict = torch.full((256 << 20,), 2, dtype=torch.int32, device=dev)
-        del evict
+        n = 512 << 20  # 2 GiB of int32, well beyond the 256 MB last-level cache
+        src = torch.ones(n, dtype=torch.int32, device=dev)
         dst = torch.empty_like(src)
+        torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
 
-    # parity of the benchmarked result against the committed golden low
-    gold = np.load(os.path.join(ROOT, "tests", "golden", "torus100.npz"))
-    ok = bool(np.array_equal(low.cpu().numpy(), gold["low"] + 1)
-              and np.array_equal(ess.cpu().numpy(), gold["ess"]))
-
-    st = stats[-1]
-    m, nnz = b.m, b.nnz
-    launches = st.phase2_launches
-    p2_bytes = 4.0 * st.merge_elems + 12.0 * m * launches
-    p2_ms = statistics.median([s.phase2_ms for s in stats])
-    dev_ms = statistics.median([s.device_ms for s in stats])
-    achieved = p2_bytes / (p2_ms * 1e-3) / 1e9 if p2_ms > 0 else 0.0
-    b_alg = 4.0 * st.merge_elems + 8.0 * nnz + 8.0 * m + 36.0 * m * (st.num_iters - 1)
-    kname = (f"k_phase2_group<{st.lanes}, {st.group_lds}>" if st.lanes > 1 else "k_phase2_lane")
-    # HBM bytes per launch of the same kernel from the committed rocprofv3
-    # counter runs (FETCH_SIZE, WRITE_SIZE scaled by a 256 MiB copy)
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "pms_torus100_traffic.json")
-    if os.path.exists(tfile):
-        tj = json.load(open(tfile))
-        if kname in tj.get("kernel", ""):
-            traffic = tj["traffic_bytes_per_launch"]
-    value = m * args.steps / wall
-    out = {
-        "metric": "column reductions/s",
-        "value": value,
-        "unit": "columns/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32",
-        "data": "synthetic",
-        "config": {"workload": "pms_torus100_vr2", "m": m, "nnz": nnz,
-                   "points": 100, "torus": "R=1 r=0.4 sigma=0.03 seed=0",
-                   "algo": "pms", "inputs": "1-based COO resident in HBM"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "kernel": kname,
-                     "bytes_per_launch": p2_bytes / max(1, launches),
-                     "avg_launch_ms": p2_ms / max(1, launches)},
-        "hbm_alg": {"bytes": b_alg, "device_ms": dev_ms,
-                    "GBps": b_alg / (dev_ms * 1e-3) / 1e9 if dev_ms > 0 else 0.0},
-        "pms": {"num_iters": st.num_iters, "additions": st.additions, "schedule": st.schedule,
-                "host_reads": st.host_reads,
-                "merge_elems": st.merge_elems, "compactions": st.compactions,
-                "pool_peak": st.pool_peak, "additions_per_s": st.additions * args.steps / wall,
-                "parity_vs_golden": ok},
-    }
-    if not args.no_twist:
+    if "dropin" in legs:
+        print("bench: drop-in with trackers", file=sys.stderr, flush=True)
+        out["dropin"] = dropin_leg(b)
+    if "cubical" in legs:
+        out["cubical64"] = cubical_leg(cpu="cpu" in legs)
+    if "twist" in legs:
         print("bench: twist on the same GPU", file=sys.stderr, flush=True)
         out["gpu_twist"] = gpu_twist_leg(b)
-    if not args.no_builders:
+    if "builders" in legs:
         print("bench: builders", file=sys.stderr, flush=True)
         out["builders"] = builder_leg()
-    if not args.no_batched:
+    if "batched" in legs:
         print("bench: batched leg", file=sys.stderr, flush=True)
         out["batched"] = batched_leg(tune=not args.no_tune)
-    if not args.no_served:
+    if "batched_only" in legs:
+        print("bench: batched call alone", file=sys.stderr, flush=True)
+        out["batched_only"] = batched_only_leg()
+    if "served" in legs:
         print("bench: served leg", file=sys.stderr, flush=True)
         out["served"] = served_leg()
-    if not args.no_cpu:
+    if "cpu" in legs and "torus" in legs:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
-        cb, cpu_low = cpu_baseline(b)
+        cb, cpu_low = cpu_baseline(b, "Torus-100")
         cb["low_sha_match"] = bool(np.array_equal(cpu_low, gold["low"]))
         out["cpu_baseline"] = cb
-        out["speedup_vs_cpu"] = value / cb["value"]
+        out["speedup_vs_cpu"] = out["value"] / cb["value"]
+    cub = out.get("cubical64")
+    if cub and "_cpu_low" in cub:
+        # the CPU twist's low against the GPU pms result of the same matrix
+        import openph_amd as O
+        low_g, _, _ = O.reduce(cub["_b"], "pms")
+        cub["cpu_baseline"]["low_sha_match"] = bool(np.array_equal(cub.pop("_cpu_low"), low_g))
+        cub.pop("_b")
+        cub.pop("cpu_equals_gpu", None)
     print(json.dumps(out))
 
 

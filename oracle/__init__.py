@@ -70,6 +70,8 @@ def _lib():
         lib.orc_pms.argtypes = [ctypes.c_int, i64p, i32p, i32p, i32p, i32p, i64p,
                                 ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_twist_dyn.restype = ctypes.c_int
+        lib.orc_twist_dyn.argtypes = [ctypes.c_int, i64p, i32p, i32p, i32p, i64p]
         lib.orc_ph_row.restype = ctypes.c_int
         lib.orc_ph_row.argtypes = [ctypes.c_int, i64p, i32p, i32p, i32p, i64p, ctypes.c_int]
         _LIB = lib
@@ -118,6 +120,18 @@ def twist(b, col_width: int = 1 << 20) -> OracleResult:
     st = np.zeros(4, np.int64)
     cw = _safe_cw(b, col_width)
     _check(_lib().orc_twist(m, colptr, rows, cw, low, ess, st), "twist")
+    return OracleResult(low, ess, st)
+
+
+def twist_dyn(b) -> OracleResult:
+    """twist on growable columns: orc_twist's order, clearing and merge with
+    no slot limit (for matrices whose fixed-slot layout is too large)."""
+    colptr, rows = _csc(b)
+    m = b.m
+    low = np.empty(m, np.int32)
+    ess = np.empty(m, np.int32)
+    st = np.zeros(4, np.int64)
+    _check(_lib().orc_twist_dyn(m, colptr, rows, low, ess, st), "twist_dyn")
     return OracleResult(low, ess, st)
 
 

@@ -18,6 +18,7 @@
  *   orc_twist     - src/cuda/algorithms/twist/twist.cu:2-87 (pseudo-dimension
  *                   descending, dims >= 1 only, clearing through low only),
  *                   pseudo-dimension = nnz(col)-1 (src/cuda/dimensions.cu:54-104).
+ *   orc_twist_dyn - the same twist on growable columns (no slot limit).
  *   orc_pms       - the pms driver src/cuda/algorithms/pms/pms.cu:1-118 with
  *                   its phases src/cuda/algorithms/pms/pms_functions.cu:7-140,
  *                   src/cuda/algorithms/kernels.cu:79-124 and left/beta
@@ -479,6 +480,51 @@ int orc_pms(int m, const int64_t *colptr, const int32_t *rows, int32_t *low,
     free(ceil_d);
 done:
     free(arglow); free(classes); free(left); free(beta); free(pd); free(visited); free(clear); free(newpiv);
+    dyn_free(&s, m);
+    return rc;
+}
+
+/* twist (twist.cu:2-87) on growable columns: the same order, clearing and
+ * merge as orc_twist, with no slot limit, for matrices whose fixed-slot
+ * layout m*p would not fit in host memory (cubical 64^3: p ~ 1,734). */
+int orc_twist_dyn(int m, const int64_t *colptr, const int32_t *rows, int32_t *low,
+                  int32_t *ess, int64_t *stats) {
+    if (m < 0) return ORC_EINPUT;
+    int rc = ORC_OK;
+    dyn_store s;
+    size_t mm = (size_t)m + 1;
+    int32_t *arglow = malloc(sizeof(int32_t) * mm), *pd = malloc(sizeof(int32_t) * mm);
+    if (dyn_init(&s, m, colptr, rows) || !arglow || !pd) { rc = ORC_ENOMEM; goto done; }
+    int cd;
+    pseudo_dims(m, colptr, pd, &cd);
+    init_low(m, colptr, rows, low);
+    for (int j = 0; j < m; j++) { arglow[j] = -1; ess[j] = 1; }
+    int64_t st[ST_N] = {0, 0, 0, 0};
+    int64_t processed = 0;
+    for (int d = cd; d > 0 && rc == ORC_OK; d--) {
+        for (int j = 0; j < m && rc == ORC_OK; j++) {
+            if (pd[j] != d) continue;
+            /* reduce_col_host (kernels_host.cu:62-75) */
+            int lj = low[j];
+            while (lj > -1 && arglow[lj] != -1 && rc == ORC_OK) {
+                ess[lj] = 0;
+                lj = dyn_add(&s, arglow[lj], j, st, &rc);
+                low[j] = lj;
+            }
+            if (rc) break;
+            if (lj > -1) {
+                arglow[lj] = j;
+                low[lj] = -1; /* twist_step_host: clear via low only */
+                ess[j] = 0;
+                ess[lj] = 0;
+            }
+            processed++;
+        }
+    }
+    st[ST_ITERS] = processed;
+    if (stats) memcpy(stats, st, sizeof(st));
+done:
+    free(arglow); free(pd);
     dyn_free(&s, m);
     return rc;
 }

@@ -304,3 +304,43 @@ def test_row_algorithms_on_random_matrices(seed, oracle_lib):
     r = oracle.ph_row(c)
     assert np.array_equal(r.low, cl) and (r.ess <= ce).all()
     assert r.additions == oracle.ph_row(c, parallel=True).additions
+
+
+@pytest.mark.parametrize("name", sorted(INDEX))
+def test_twist_on_growable_columns_equals_fixed_slot_twist(name, oracle_lib):
+    """orc_twist_dyn (the CPU baseline for matrices whose fixed-slot layout
+    does not fit in memory) takes the same columns in the same order and adds
+    the same pivots as orc_twist: outputs and counters equal."""
+    f = np.load(os.path.join(GOLD, name + ".npz"))
+    b = _gen(name)
+    d = oracle.twist_dyn(b)
+    t = oracle.twist(b)
+    assert np.array_equal(d.low, f["low"]) and np.array_equal(d.ess, f["ess"])
+    assert (d.additions, d.merge_elems, d.max_len, d.iters) == \
+        (t.additions, t.merge_elems, t.max_len, t.iters)
+
+
+def test_digest_fixtures_are_consistent():
+    """The digest fixtures of the batched and served workloads name the
+    matrices the generators still produce (checked on the small members)."""
+    import hashlib
+    import json
+
+    def coo_digest(b):
+        r, c = b.coo1()
+        h = hashlib.sha256()
+        h.update(np.int64(b.m).tobytes())
+        h.update(np.ascontiguousarray(r, dtype="<i4").tobytes())
+        h.update(np.ascontiguousarray(c, dtype="<i4").tobytes())
+        return h.hexdigest()[:16]
+
+    mix = json.load(open(os.path.join(GOLD, "batched512.json")))["members"]
+    assert len(mix) == 512 and sum(x["m"] for x in mix) == 14757723
+    mats = C.batched_mix(8, seed=0)  # the first children of the same seed sequence
+    for b, g in zip(mats, mix):
+        assert b.m == g["m"] and coo_digest(b) == g["coo"]
+        r = oracle.pms(b, snapshot=True)
+        assert hashlib.sha256(r.low.astype("<i4").tobytes()).hexdigest()[:16] == g["low"]
+    served = json.load(open(os.path.join(GOLD, "served17.json")))["requests"]
+    kinds = [(s["kind"], s["size"]) for s in served]
+    assert ("points", 190) in kinds and ("field", 85) in kinds and ("field", 81) in kinds
