@@ -88,6 +88,9 @@ struct Dev {
     int nnz = 0;
     int ntiles = 0;
     int slot0 = 8;      // minimum segment size of a column in the pool
+    // Phase II worklists at most this long (0 = never) are taken by
+    // 256-lane teams (k_phase2_wide) instead of the 1024-entry lane groups
+    int wide_max = 0;
     // per column
     int *low = nullptr, *arglow = nullptr, *classes = nullptr, *ess = nullptr;
     int *left = nullptr, *beta = nullptr, *first = nullptr, *clear = nullptr;

@@ -189,6 +189,7 @@ struct RunCfg {
     bool track = false;
     bool timing = true;
     bool no_alloc = false;  // the context's buffers only: see run_pms
+    bool wide = true;       // 256-lane teams for short worklists of long columns
 };
 
 RunCfg cfg_from(const openph_options *opt, int nnz);

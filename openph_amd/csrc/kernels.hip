@@ -1132,6 +1132,7 @@ __device__ __forceinline__ int mp_walk_lds_ahead(const int *a, int na, const int
 template <int G>
 struct WaveTeam {
     static constexpr int size = G;
+    static constexpr bool wave = true;
     int gl, gw;
     __device__ int lane() const { return gl; }
     __device__ void sync() const { __syncwarp(); }
@@ -1185,6 +1186,7 @@ struct WaveTeam {
 template <int BT>
 struct BlockTeam {
     static constexpr int size = BT;
+    static constexpr bool wave = false;
     int *s_scan;                 // BT / 64 ints of LDS
     unsigned long long *s_word;  // one word of LDS
     __device__ int lane() const { return threadIdx.x; }
@@ -1249,42 +1251,97 @@ __device__ int team_xor_lds(const Team &t, int *a, int na, const int *b, int nb,
     return total;
 }
 
-// The same sum with the result in b's buffer: survivors go to provisional
-// slots of c at their first diagonal, and after the scan (every walk is over)
-// to their places in b, followed by a new sentinel. a may be in LDS (LDS_A,
-// with its sentinel) or in the pool; b[nb] holds the sentinel kIntMax, so
-// n + 1 entries of b's buffer are used.
+// The same sum with the result in b's buffer, in two parts so that a caller
+// can issue global loads between them. xor_walk: survivors go to provisional
+// slots of c at their first diagonal, the team scan gives their places;
+// returns the sum's length, with the lane's survivor count, first diagonal
+// and output place in w. xor_copy (after a team barrier: every walk is over):
+// the survivors to their places in b, followed by a new sentinel. a may be in
+// LDS (LDS_A, with its sentinel) or in the pool; b[nb] holds the sentinel
+// kIntMax, so n + 1 entries of b's buffer are used.
+struct XorWalk {
+    int cnt, d0, ex, total;
+};
+
 template <bool LDS_A, class Team>
-__device__ int team_xor_into_b(const Team &t, const int *a, int na, int *b, int nb, int *c,
-                               int spare) {
+__device__ __forceinline__ XorWalk xor_walk(const Team &t, const int *a, int na, const int *b,
+                                            int nb, int *c) {
     const int n = na + nb;
     const int K = (int)(((unsigned)n + Team::size - 1) / Team::size);
     const int l = t.lane();
-    const int d0 = l * K < n ? l * K : n;
-    const int d1 = d0 + K < n ? d0 + K : n;
-    int cnt;
-    if (!LDS_A) cnt = mp_walk<true>(a, na, b, nb, d0, d1, c + d0);
-    else if constexpr (Team::size >= 32) cnt = mp_walk_lds_ahead(a, na, b, nb, d0, d1, c + d0);
-    else cnt = mp_walk_lds<true>(a, na, b, nb, d0, d1, c + d0);
-    int total;
-    const int ex = t.scan(cnt, total);
-    t.sync();
-    // four reads, then four writes, without branches: reads past the lane's
-    // survivors are clamped to c[spare] and their writes go there (spare >= n
-    // is a slot of c that no walk uses)
-    for (int q = 0; q < cnt; q += 4) {
-        const int s0 = d0 + q;
+    XorWalk w;
+    w.d0 = l * K < n ? l * K : n;
+    const int d1 = w.d0 + K < n ? w.d0 + K : n;
+    if (!LDS_A) w.cnt = mp_walk<true>(a, na, b, nb, w.d0, d1, c + w.d0);
+    else if constexpr (Team::size >= 32) w.cnt = mp_walk_lds_ahead(a, na, b, nb, w.d0, d1, c + w.d0);
+    else w.cnt = mp_walk_lds<true>(a, na, b, nb, w.d0, d1, c + w.d0);
+    w.ex = t.scan(w.cnt, w.total);
+    return w;
+}
+
+// four reads, then four writes, without branches: reads past the lane's
+// survivors are clamped to c[spare] and their writes go there (spare >= n is
+// a slot of c that no walk uses)
+__device__ __forceinline__ void xor_copy(const XorWalk &w, int *b, int *c, int spare) {
+    for (int q = 0; q < w.cnt; q += 4) {
+        const int s0 = w.d0 + q;
         const int x0 = c[s0];
-        const int x1 = c[q + 1 < cnt ? s0 + 1 : spare];
-        const int x2 = c[q + 2 < cnt ? s0 + 2 : spare];
-        const int x3 = c[q + 3 < cnt ? s0 + 3 : spare];
-        b[ex + q] = x0;
-        (q + 1 < cnt ? b + ex + q + 1 : c + spare)[0] = x1;
-        (q + 2 < cnt ? b + ex + q + 2 : c + spare)[0] = x2;
-        (q + 3 < cnt ? b + ex + q + 3 : c + spare)[0] = x3;
+        const int x1 = c[q + 1 < w.cnt ? s0 + 1 : spare];
+        const int x2 = c[q + 2 < w.cnt ? s0 + 2 : spare];
+        const int x3 = c[q + 3 < w.cnt ? s0 + 3 : spare];
+        b[w.ex + q] = x0;
+        (q + 1 < w.cnt ? b + w.ex + q + 1 : c + spare)[0] = x1;
+        (q + 2 < w.cnt ? b + w.ex + q + 2 : c + spare)[0] = x2;
+        (q + 3 < w.cnt ? b + w.ex + q + 3 : c + spare)[0] = x3;
     }
-    b[total] = kIntMax;  // every lane: the same value
-    return total;
+    b[w.total] = kIntMax;  // every lane: the same value
+}
+
+template <bool LDS_A, class Team>
+__device__ int team_xor_into_b(const Team &t, const int *a, int na, int *b, int nb, int *c,
+                               int spare) {
+    const XorWalk w = xor_walk<LDS_A>(t, a, na, b, nb, c);
+    t.sync();
+    xor_copy(w, b, c, spare);
+    return w.total;
+}
+
+// The largest element of a xor b for ascending lists whose largest elements
+// are equal (a column and the pivot of its low): the tops cancel, and so do
+// the elements below them while both lists agree. -1 when the sum is empty.
+// Every lane of a team reads the same entries and gets the same value.
+__device__ __forceinline__ int xor_top_from(const int *a, int na, const int *b, int nb, int q) {
+    int i = na - 2 - q, k = nb - 2 - q;
+    while (true) {
+        const int x = i >= 0 ? a[i] : -1;
+        const int y = k >= 0 ? b[k] : -1;
+        if (x != y) return x > y ? x : y;
+        if (x < 0) return -1;
+        i--;
+        k--;
+    }
+}
+
+// The same for a team of one whole wave: lane q compares the q-th pair below
+// the tops, one LDS round trip for the first 64 pairs.
+template <class Team>
+__device__ __forceinline__ int xor_top(const Team &t, const int *a, int na, const int *b, int nb) {
+    (void)t;
+    if constexpr (Team::size >= 64) {
+        // a whole wave, or every wave of a block team (each computes the
+        // same value from the same entries)
+        const int q = lane_id();
+        const int i = na - 2 - q, k = nb - 2 - q;
+        const int x = i >= 0 ? a[i] : -1;
+        const int y = k >= 0 ? b[k] : -1;
+        const unsigned long long ne = __ballot(x != y || x < 0);
+        if (!ne) return xor_top_from(a, na, b, nb, 64);
+        const int qs = __builtin_amdgcn_readfirstlane(__builtin_ctzll(ne));
+        const int xs = __builtin_amdgcn_readlane(x, qs), ys = __builtin_amdgcn_readlane(y, qs);
+        return xs > ys ? xs : ys;  // both -1 when the sum is empty
+    } else {
+        return xor_top_from(a, na, b, nb, 0);
+    }
 }
 
 // Allocate `n` pool entries by the team leader; ~0ull when the pool is full.
@@ -1338,29 +1395,89 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
         sB[clen] = kIntMax;
     }
     t.sync();
+    // Teams of 32 lanes and more take the long columns, whose pivots are
+    // mostly short. The next low of a sum is known from the tops of the two
+    // lists before the merge (xor_top), so such a team loads the next
+    // descriptor while it walks the merge and the next pivot's entries (at
+    // most PFR per lane, into registers) while it copies the survivors back:
+    // a step then waits for one global load, after the merge, instead of two
+    // in a row before it. Pivots read in a Phase II pass are not written
+    // during it, so the early loads see what the late ones would.
+    constexpr bool PF = G >= 32;
+    constexpr int PFR = 4;
+    int pa[PFR];
+    bool have = false;  // pa holds the entries of pd's pivot
     // one exit, at the top
     while (!(fl & (kHand | kStop)) && pd.x > -1 && pd.x < j) {
         const int na = pd.z;
         const unsigned aoff = (unsigned)pd.y;
+        const bool pre = PF && have;  // this step's pivot is in pa
+        have = false;
+        bool nxt = false;  // pdn = pivdesc[nl], loaded during this step
+        // written only by its load (and read only when nxt is set): a write
+        // here would make the compiler wait for every load in flight
+        int4 pdn;
         int nc = 0, nl = 0;
         if (!(fl & kGlob) && na + clen < W) {
-            const bool ain = na < WA;
+            // WA >= W: every pivot of a sum that fits in LDS fits sA
+            const bool ain = WA >= W || na < WA;
             if (ain) {
-                for (int i = gl; i < na; i += G) sA[i] = d.pool[aoff + i];
+                if (pre) {
+#pragma unroll
+                    for (int r = 0; r < PFR; r++) {
+                        const int i = gl + G * r;
+                        if (i < na) sA[i] = pa[r];
+                    }
+                } else {
+                    for (int i = gl; i < na; i += G) sA[i] = d.pool[aoff + i];
+                }
                 sA[na] = kIntMax;
                 t.sync();
             }
-            // Stores go after the loads they would otherwise delay: a wait
-            // for a load also waits for every older store. Every lane of the
-            // team stores (the same value to the same address), so that no
-            // branch hides the store from the compiler's wait counting.
-            d.ess[lj] = 0;
-            if (ain)
-                nc = team_xor_into_b<true>(t, sA, na, sB, clen, sC, W - 1);
-            else
-                nc = team_xor_into_b<false>(t, d.pool + aoff, na, sB, clen, sC, W - 1);
-            t.sync();
-            nl = nc ? sB[nc - 1] : -1;
+            if (PF && ain) {
+                // The early loads are unconditional (a valid row when there
+                // is no next low) and their registers are not written on any
+                // other path: a write to a register with a load in flight
+                // makes the compiler wait for that load at once.
+                const int nlp = xor_top(t, sA, na, sB, clen);
+                nxt = nlp > -1 && nlp < lj;
+                pdn = d.pivdesc[nxt ? nlp : lj];
+                const XorWalk w = xor_walk<true>(t, sA, na, sB, clen, sC);
+                nc = w.total;
+                // the next pivot's entries, when the chain goes on with it;
+                // lanes past its end read its last entry
+                if (nxt && pdn.x > -1 && pdn.x < j && pdn.w != j && pdn.z <= PFR * G) {
+                    const int nr = (pdn.z + G - 1) / G;
+#pragma unroll
+                    for (int r = 0; r < PFR; r++) {
+                        const int i = gl + G * r;
+                        if (r < nr) pa[r] = d.pool[(unsigned)pdn.y + (i < pdn.z ? i : pdn.z - 1)];
+                    }
+                    have = true;
+                }
+                t.sync();
+                xor_copy(w, sB, sC, W - 1);
+                // Stores go after the loads they would otherwise delay: a
+                // wait for a load also waits for every older store. Every
+                // lane of the team stores (the same value to the same
+                // address), so that no branch hides the store from the
+                // compiler's wait counting.
+                d.ess[lj] = 0;
+                t.sync();
+                nl = nc ? sB[nc - 1] : -1;
+                if (nl != nlp) {  // the scan and the merge must agree
+                    if (gl == 0) atomicOr(&d.ctrl->err, kErrState);
+                    fl |= kStop;
+                }
+            } else {
+                d.ess[lj] = 0;
+                if (ain)
+                    nc = team_xor_into_b<true>(t, sA, na, sB, clen, sC, W - 1);
+                else
+                    nc = team_xor_into_b<false>(t, d.pool + aoff, na, sB, clen, sC, W - 1);
+                t.sync();
+                nl = nc ? sB[nc - 1] : -1;
+            }
         } else if (HANDOFF) {
             fl |= kHand;
         } else {
@@ -1395,10 +1512,16 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
                 lj = nl;
                 if (lj < 0) {
                     fl |= kStop;
-                } else {
-                    pd = d.pivdesc[lj];  // .w = left[lj]: one load for the rule and the pivot
-                    d.clear[lj] = 1;
+                } else if (nxt) {
+                    pd = pdn;  // loaded during the merge
                     if (pd.w == j) fl |= kPivot | kStop;
+                    d.clear[lj] = 1;
+                } else {
+                    // .w = left[lj]: one load for the rule and the pivot, used
+                    // here so that its wait stays on this path
+                    pd = d.pivdesc[lj];
+                    if (pd.w == j) fl |= kPivot | kStop;
+                    d.clear[lj] = 1;
                 }
             }
         }
@@ -1483,6 +1606,7 @@ __global__ void __launch_bounds__(kBlock) k_phase2_group(Dev d) {
     int *const buf = lds + gid * S::PER;
     if (d.ctrl->halt) return;
     const int nwork = d.ctrl->nwork;
+    if (nwork <= d.wide_max) return;  // the wide pass takes short worklists
     Phase2Counters ctr;
     for (int w = blockIdx.x * NG + gid; w < nwork; w += gridDim.x * NG)
         phase2_column<WaveTeam<G>, W, S::WA, true>(t, d, d.work[w], buf, ctr);
@@ -1535,6 +1659,29 @@ __global__ void __launch_bounds__(kLongTeam) k_phase2_long(Dev d) {
     Phase2Counters ctr;
     for (int w = blockIdx.x; w < nlong; w += gridDim.x)
         phase2_column<BlockTeam<kLongTeam>, kLongW, kLongW, false>(t, d, d.longw[w], lds, ctr);
+    if (threadIdx.x == 0) {
+        if (ctr.ovf) d.ctrl->overflow = 1;
+        stripe_add(d.ctrl, ctr.nadd, ctr.nel, ctr.spills);
+    }
+}
+
+// Short worklists (at most Dev::wide_max columns, the late iterations of
+// reductions whose columns grow long): the GPU holds every column at once,
+// so each gets a team of kLongTeam lanes with 4096-entry buffers instead of
+// one wave: a merge of n entries walks n / 256 diagonals per lane instead of
+// n / 64, and the chains of dependent additions, which set the pass's time,
+// get shorter steps.
+__global__ void __launch_bounds__(kLongTeam) k_phase2_wide(Dev d) {
+    __shared__ int lds[3 * kLongW];
+    __shared__ int s_scan[kLongTeam / 64];
+    __shared__ unsigned long long s_word;
+    const BlockTeam<kLongTeam> t{s_scan, &s_word};
+    if (d.ctrl->halt) return;
+    const int nwork = d.ctrl->nwork;
+    if (nwork > d.wide_max) return;
+    Phase2Counters ctr;
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x)
+        phase2_column<BlockTeam<kLongTeam>, kLongW, kLongW, false>(t, d, d.work[w], lds, ctr);
     if (threadIdx.x == 0) {
         if (ctr.ovf) d.ctrl->overflow = 1;
         stripe_add(d.ctrl, ctr.nadd, ctr.nel, ctr.spills);
@@ -1642,10 +1789,12 @@ void launch_phase2(const Dev &d, int lanes, int wlds, int grid, int long_grid, h
     k<<<grid, kBlock, 0, s>>>(d);
     // handed-off columns: wave teams when the groups' buffers are smaller
     // than theirs, else 256-lane teams (three 48 KB blocks per CU each)
-    if (wlds < kMidW)
+    if (wlds < kMidW) {
         k_phase2_mid<<<3 * long_grid, kBlock, 0, s>>>(d);
-    else
+    } else {
         k_phase2_long<<<3 * long_grid, kLongTeam, 0, s>>>(d);
+        if (d.wide_max) k_phase2_wide<<<3 * long_grid, kLongTeam, 0, s>>>(d);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -456,6 +456,9 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
         sh.grid = 2 * phase2_blocks_per_cu(sh.lanes, sh.wlds) * c.num_cu;
         sh.long_grid = c.num_cu;
     }
+    // long columns (the 1024-entry shapes): iterations whose worklist fits
+    // the GPU's 256-lane teams at once (three per CU) use them
+    d.wide_max = sh.wlds >= 1024 && cfg.wide ? 3 * c.num_cu : 0;
     const bool timed = stats != nullptr && events;
 
     if (events) OPH_CHECK(hipEventRecord(c.ev0, s));
