@@ -566,6 +566,7 @@ def _served_run(pool, coo, ref, seq, cols, qps, workers, rng):
     arrivals = (np.zeros(n_requests) if qps is None
                 else np.cumsum(rng.exponential(1.0 / qps, n_requests)))
     tickets = [None] * n_requests
+    kinds = []
     with Server(workers=workers) as srv:
         def submitter():
             start = time.perf_counter()
@@ -579,23 +580,36 @@ def _served_run(pool, coo, ref, seq, cols, qps, workers, rng):
         th = threading.Thread(target=submitter)
         th.start()
         lat, first, last, same = [], None, None, True
+        last_submit = None
         for k, i in enumerate(seq):
             while tickets[k] is None:
                 time.sleep(0.0002)
             low, ess, info = srv.wait(tickets[k])
             same &= bool(np.array_equal(low, ref[i][0]) and np.array_equal(ess, ref[i][1]))
             lat.append(info.done_s - info.submit_s)
+            kinds.append(pool[i].m)
             first = info.submit_s if first is None else min(first, info.submit_s)
             last = info.done_s if last is None else max(last, info.done_s)
+            last_submit = info.submit_s if last_submit is None else max(last_submit, info.submit_s)
         th.join()
         st = srv.stats()
     span = last - first
     lat_ms = np.array(lat) * 1e3
+    # latency per request class: the VR 2-skeleta (input columns of at most
+    # 3 entries) and the cubical grids
+    cls = np.array([int(np.diff(pool[i].colptr).max(initial=0) > 4) for i in seq])
+    by = {}
+    for c, name in ((0, "vr"), (1, "cubical")):
+        x = lat_ms[cls == c]
+        if x.size:
+            by[name] = {"requests": int(x.size), "latency_ms_p50": float(np.percentile(x, 50)),
+                        "latency_ms_p99": float(np.percentile(x, 99))}
     return {
         "qps": qps, "served_cols_per_s": cols / span, "span_s": span,
+        "submit_span_s": last_submit - first,
         "latency_ms_p50": float(np.percentile(lat_ms, 50)),
         "latency_ms_p99": float(np.percentile(lat_ms, 99)),
-        "latency_ms_max": float(lat_ms.max()),
+        "latency_ms_max": float(lat_ms.max()), "by_class": by,
         "batches": st["batches"], "served_equals_alone": same,
     }
 

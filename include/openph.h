@@ -239,9 +239,15 @@ int openph_autotune_batch_device(int count, const int *ms, const int *nnzs, cons
 /*
  * Served reduction (no counterpart in the reference, which is a blocking
  * MATLAB call). A server owns `workers` device workspaces, each with its own
- * HIP stream and thread; queued requests are batched onto a free worker while
- * their pool footprint fits worker_bytes, and batches of different workers
- * run concurrently on the device.
+ * HIP stream and thread, sized at creation for worker_bytes. Each worker keeps
+ * one running batch of requests that are reduced by the same pms iterations:
+ * between chunks of two iterations it releases every request whose own lows
+ * no longer repeat, and it admits queued requests of the batch's class
+ * (input columns of at most 4 entries, or longer) while their columns, COO
+ * entries and pool footprint fit its budget and the batch keeps at most
+ * OPENPH_MAX_DIMS distinct column lengths. Batches of different workers run
+ * concurrently on the device. A request too large for an empty batch runs
+ * alone; the members of a batch that fails are run again alone.
  */
 typedef struct openph_server openph_server;
 
@@ -257,18 +263,20 @@ typedef struct openph_server_options {
 
 typedef struct openph_request_info {
     double submit_s;         /* seconds on the server's steady clock         */
-    double start_s;          /* its batch started                            */
-    double done_s;           /* its batch finished (results on the host)     */
-    int batch_size;          /* requests in its batch                        */
+    double start_s;          /* it joined a batch                            */
+    double done_s;           /* its results were on the host                 */
+    int batch_size;          /* requests that shared its batch up to its
+                                release                                       */
     int worker;              /* worker that ran it                           */
-    int num_iters;           /* num_iters of its batch                       */
+    int num_iters;           /* iterations it took part in                   */
     int status;              /* 0 or a negative OPENPH_E* code               */
     int m;                   /* columns of the request                       */
 } openph_request_info;
 
 int openph_server_create(const openph_server_options *opt, openph_server **out);
 /* Queue one matrix (host 1-based COO, copied); returns a ticket. Never waits
- * for the device; OPENPH_EBUSY when the queue is full. */
+ * for the device; OPENPH_EBUSY when the queue is full, OPENPH_EDIMS when its
+ * columns have more than OPENPH_MAX_DIMS distinct lengths. */
 int openph_server_submit(openph_server *srv, const int *rows1, const int *cols1, int m, int nnz,
                          int64_t *ticket);
 /* Queue a matrix the worker builds on the device from its input (host
@@ -281,7 +289,8 @@ int openph_server_submit_vr(openph_server *srv, const double *points, int n, int
 int openph_server_submit_cubical(openph_server *srv, const float *field, int s0, int s1, int s2,
                                  int64_t *ticket);
 /* Block until the ticket's reduction is done; low1/ess (host, m entries) get
- * its outputs. A ticket can be waited for once. */
+ * its outputs. A ticket can be waited for once: a second wait, also one that
+ * starts while the first blocks, returns OPENPH_EINVAL. */
 int openph_server_wait(openph_server *srv, int64_t ticket, int *low1, int *ess,
                        openph_request_info *info);
 int openph_server_stats(openph_server *srv, int64_t *queued, int64_t *batches,

@@ -104,6 +104,12 @@ struct Dev {
     int *tile_max = nullptr, *tile_pre = nullptr;
     int *pool = nullptr, *pool2 = nullptr;   // column pool (int32 rows)
     Ctrl *ctrl = nullptr;
+    // a running batch of matrices (the server): column ranges of its members
+    // [mstart[k], mstart[k+1]) and, per iteration, mdup[k] = 1 when a low of
+    // member k repeats (null / 0 outside the server)
+    int *mdup = nullptr;
+    const int *mstart = nullptr;
+    int nmem = 0;
     // per-iteration trackers and iteration times of the queued / graph
     // schedules, read once per chunk (null when not wanted)
     const int *ltrue = nullptr;              // 0-based low_true
@@ -139,6 +145,19 @@ void launch_apply1(const Dev &d, hipStream_t s);
 // 256-lane teams). grid = blocks of the groups; long_grid = CUs for the
 // long-column pass (three of its blocks per CU).
 void launch_phase2(const Dev &d, int lanes, int wlds, int grid, int long_grid, hipStream_t s);
+// A running batch (the server): columns [a, d.m) of members appended to it,
+// with their global 1-based COO entries [e0, e1): per-column state and pool
+// layout after the current pool top, validation, left/beta, slots from the
+// host's slot map (nslots of them), Phase 0 and its clears.
+void launch_setup_range(const Dev &d, int a, int e0, int e1, const int *rows1, const int *cols1,
+                        hipStream_t s);
+void launch_slots_range(const Dev &d, int a, int nslots, hipStream_t s);
+void launch_phase0_range(const Dev &d, int a, hipStream_t s);
+// A member [a, b) whose lows no longer repeat: ess_hat_final on its range,
+// its 1-based local low (low - a + 1, 0 = none) and ess into low1/ess, then
+// its columns made inert (low -1, cleared, no pivots) so that later
+// iterations and compactions skip them.
+void launch_retire_range(const Dev &d, int a, int b, int *low1_out, int *ess_out, hipStream_t s);
 // Batches: local 1-based COO of `count` matrices -> global block-diagonal
 // COO (err gets bit 1 for an index outside its matrix), and global 1-based
 // low -> local 1-based low.

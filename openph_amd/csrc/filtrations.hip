@@ -13,6 +13,8 @@
 
 #include <cstring>
 
+#include <algorithm>
+
 #include "host.h"
 
 namespace oph {
@@ -278,6 +280,30 @@ int scan_offsets(Context &c, const int *nnz, int *off, long long m, hipStream_t 
 }
 
 }  // namespace
+
+// Grows the builders' buffers for matrices of up to m columns (and VR
+// complexes of up to n points), so that serving them does not allocate:
+// hipFree waits for the whole device.
+int reserve_builders(Context &c, size_t m, int n) {
+    if (m == 0 || m > 0x7fffffffull) return OPENPH_OK;
+    OPH_CHECK(grow(&c.f_dist, c.c_fdist, (size_t)n * n + 1));
+    OPH_CHECK(grow(&c.f_key, c.c_fkey, m));
+    OPH_CHECK(grow(&c.f_key2, c.c_fkey2, m));
+    OPH_CHECK(grow(&c.f_val, c.c_fval, m));
+    OPH_CHECK(grow(&c.f_val2, c.c_fval2, m));
+    OPH_CHECK(grow(&c.f_pos, c.c_fpos, m));
+    OPH_CHECK(grow(&c.f_nnz, c.c_fnnz, m));
+    OPH_CHECK(grow(&c.f_off, c.c_foff, m));
+    size_t b1 = 0, b2 = 0, b3 = 0;
+    OPH_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, c.f_key, c.f_key2, c.f_val, c.f_val2,
+                                                 (int)m, 0, 64, (hipStream_t)0));
+    OPH_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, b2, c.f_key, c.f_key2, (int)m, 0, 64,
+                                                (hipStream_t)0));
+    OPH_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, c.f_nnz, c.f_off, (int)m,
+                                               (hipStream_t)0));
+    OPH_CHECK(grow(&c.f_tmp, c.c_ftmp, std::max(b1, std::max(b2, b3)) + 1));
+    return OPENPH_OK;
+}
 
 // The two builders on a given workspace and stream, without waiting for
 // them (the server's workers build requests inside their batches).

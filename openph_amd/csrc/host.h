@@ -193,6 +193,55 @@ struct RunCfg {
 };
 
 RunCfg cfg_from(const openph_options *opt, int nnz);
+
+// A batch of matrices that members join and leave between chunks of
+// iterations (the server's workers). Members are appended at the end of the
+// block-diagonal matrix; one whose lows no longer repeat is output and its
+// columns made inert, and the batch ends when none is left. All members share
+// one Phase II shape: class 0 for matrices whose input columns hold at most 4
+// entries (simplicial complexes up to dimension 3), class 1 otherwise.
+struct LiveBatch {
+    bool running = false;
+    int cls = 0;
+    int lanes = 0, wlds = 0, grid = 0, long_grid = 0;  // Phase II shape
+    std::vector<int> mstart;       // member k owns columns [mstart[k], mstart[k+1])
+    std::vector<int> slot_of_len;  // input column length -> pseudo-dimension slot
+    int nslots = 0;
+    size_t lcap = 0;               // logical pool capacity
+    int compactions = 0;
+    int *d_mstart = nullptr, *d_mdup = nullptr, *h_mdup = nullptr;
+    size_t c_mstart = 0, c_mdup = 0, c_hmdup = 0;
+    LiveBatch() = default;
+    LiveBatch(const LiveBatch &) = delete;
+    LiveBatch &operator=(const LiveBatch &) = delete;
+    ~LiveBatch() {
+        if (d_mstart) (void)hipFree(d_mstart);
+        if (d_mdup) (void)hipFree(d_mdup);
+        if (h_mdup) (void)hipHostFree(h_mdup);
+    }
+};
+// Starts an empty batch of class cls on the workspace, with room for
+// max_members members and a pool of pool_elems entries.
+int live_start(Context &c, LiveBatch &lb, const RunCfg &cfg, int cls, int max_members,
+               size_t pool_elems, hipStream_t s);
+// Appends `count` members: ms/nnzs their sizes, lens[k] the distinct column
+// lengths of member k's input (their union with the batch's must stay within
+// kMaxSlots), rows1/cols1 their local 1-based COO concatenated on the device
+// (nnz entries). The workspace must have room for the columns and entries.
+int live_append(Context &c, LiveBatch &lb, int count, const int *ms, const int *nnzs,
+                const std::vector<int> *lens, const int *rows1, const int *cols1, int nnz,
+                hipStream_t s);
+// Queues `iters` iterations and reads the control block and the members'
+// duplicate flags (lb.h_mdup) back; compacts the pool when asked. Returns an
+// OPENPH_E* code when the reduction failed; c.h_ctrl->halt is
+// kHaltConverged when no low of any member repeats.
+int live_chunk(Context &c, LiveBatch &lb, const RunCfg &cfg, int iters, hipStream_t s);
+// Member k's 1-based local low and ess into low1/ess (device), its columns
+// made inert; trailing retired members' columns are given back.
+int live_retire(Context &c, LiveBatch &lb, int k, int *low1, int *ess, hipStream_t s);
+// Drops retired members at the end of the batch (their columns can be
+// reused); `retired` flags every member.
+void live_trim(Context &c, LiveBatch &lb, const std::vector<char> &retired, hipStream_t s);
 // The workspace of a device, created on first use and kept for the process.
 Context *context_for(int device);
 // Filtrations built on the device into rows1/cols1 (1-based COO), queued on
@@ -205,6 +254,9 @@ int build_cubical(Context &c, const float *field, int s0, int s1, int s2, int *r
 // pool_elems entries, so that serving does not allocate (hipFree waits for
 // the whole device).
 int reserve(Context &c, size_t m, size_t nnz, size_t pool_elems);
+// The same for the filtration builders: matrices of up to m columns built
+// from up to n points (filtrations.hip).
+int reserve_builders(Context &c, size_t m, int n);
 int set_device(int device);
 // The pms reduction of one matrix held on the device (1-based COO); low1/ess
 // are device outputs, tr (may be null) the drop-in's trackers.

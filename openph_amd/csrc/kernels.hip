@@ -159,9 +159,10 @@ __device__ __forceinline__ unsigned col_size(const Dev &d, int j, int mode) {
     return slot_size(n, d.slot0);
 }
 
-__global__ void __launch_bounds__(kBlock) k_scan_tiles(Dev d, int mode) {
+// The scan runs over columns [a, d.m) in tiles of kTile.
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(Dev d, int mode, int a) {
     __shared__ unsigned long long lds[kBlock / 64];
-    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
+    const int base = a + blockIdx.x * kTile + threadIdx.x * kItems;
     unsigned long long sum = 0;
 #pragma unroll
     for (int k = 0; k < kItems; k++) sum += col_size(d, base + k, mode);
@@ -170,26 +171,31 @@ __global__ void __launch_bounds__(kBlock) k_scan_tiles(Dev d, int mode) {
     if (threadIdx.x == 0) d.scan_tmp[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(1024) k_scan_partials(Dev d) {
+// nt tiles; with append the offsets start at the current pool top, which
+// then moves past them (columns added to a running batch).
+__global__ void __launch_bounds__(1024) k_scan_partials(Dev d, int nt, int append) {
     __shared__ unsigned long long lds[16];
-    unsigned long long running = 0;
-    for (int c0 = 0; c0 < d.ntiles; c0 += blockDim.x) {
+    __shared__ unsigned long long s_base;
+    if (threadIdx.x == 0) s_base = append ? d.ctrl->pool_top : 0ull;
+    __syncthreads();
+    unsigned long long running = s_base;
+    for (int c0 = 0; c0 < nt; c0 += blockDim.x) {
         int t = c0 + threadIdx.x;
-        unsigned long long v = t < d.ntiles ? d.scan_tmp[t] : 0ull;
+        unsigned long long v = t < nt ? d.scan_tmp[t] : 0ull;
         unsigned long long tot;
         unsigned long long ex = block_exclusive_scan(v, SumOp<unsigned long long>(), 0ull, lds, &tot);
-        if (t < d.ntiles) d.scan_tmp[t] = ex + running;
+        if (t < nt) d.scan_tmp[t] = ex + running;
         running += tot;
     }
     if (threadIdx.x == 0) {
-        d.scan_tmp[d.ntiles] = running;
+        d.scan_tmp[nt] = running;
         d.ctrl->pool_top = running;
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_scan_apply(Dev d, int mode, unsigned *out) {
+__global__ void __launch_bounds__(kBlock) k_scan_apply(Dev d, int mode, unsigned *out, int a) {
     __shared__ unsigned long long lds[kBlock / 64];
-    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
+    const int base = a + blockIdx.x * kTile + threadIdx.x * kItems;
     unsigned v[kItems];
     unsigned long long sum = 0;
 #pragma unroll
@@ -211,10 +217,13 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(Dev d, int mode, unsigned
     }
 }
 
-static void launch_scan(const Dev &d, int mode, unsigned *out, hipStream_t s) {
-    k_scan_tiles<<<d.ntiles, kBlock, 0, s>>>(d, mode);
-    k_scan_partials<<<1, 1024, 0, s>>>(d);
-    k_scan_apply<<<d.ntiles, kBlock, 0, s>>>(d, mode, out);
+static void launch_scan(const Dev &d, int mode, unsigned *out, hipStream_t s, int a = 0,
+                        bool append = false) {
+    const int nt = (d.m - a + kTile - 1) / kTile;
+    if (nt <= 0) return;
+    k_scan_tiles<<<nt, kBlock, 0, s>>>(d, mode, a);
+    k_scan_partials<<<1, 1024, 0, s>>>(d, nt, append ? 1 : 0);
+    k_scan_apply<<<nt, kBlock, 0, s>>>(d, mode, out, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -312,15 +321,17 @@ __device__ __forceinline__ int lower_bound_i(const int *a, int n, int key) {
 // from its right neighbour lane and each lane searches once. The longest
 // column reaches Ctrl::maxlen with one atomic per block: one per lane on that
 // single address took 23 us of a Torus-100 setup.
-__global__ void k_init_cols(Dev d, const int *__restrict__ rows1, const int *__restrict__ cols1) {
+// Columns [a, d.m) and their COO entries [e0, e1) (global 1-based indices).
+__global__ void k_init_cols(Dev d, const int *__restrict__ rows1, const int *__restrict__ cols1,
+                            int a, int e0, int e1) {
     const int lane = lane_id();
     const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long long jl = w * 63 + lane;
+    const long long jl = a + w * 63 + lane;
     const int key = jl < d.m ? (int)jl + 1 : d.m + 1;
     __shared__ int s_max;
     if (threadIdx.x == 0) s_max = 0;
     __syncthreads();
-    const int lo = lower_bound_i(cols1, d.nnz, key);
+    const int lo = e0 + lower_bound_i(cols1 + e0, e1 - e0, key);
     int hi = __shfl_down(lo, 1, 64);
     const bool live = lane != 63 && jl < d.m;
     if (hi < lo) hi = lo;  // unsorted input; flagged by k_entries
@@ -363,8 +374,11 @@ static int init_cols_blocks(int m) {
 // issued once per distinct row and block.
 constexpr int kEntriesBlock = kTile;  // one entry per thread
 
+// Entries [e0, e1), of columns [a, d.m): a row below a (another matrix of
+// a batch) is out of range.
 __global__ void __launch_bounds__(kEntriesBlock) k_entries(Dev d, const int *__restrict__ rows1,
-                                                          const int *__restrict__ cols1) {
+                                                          const int *__restrict__ cols1, int a,
+                                                          int e0, int e1) {
     constexpr int H = 2 * kTile;
     __shared__ int hkey[H];
     __shared__ int hval[H];
@@ -375,12 +389,12 @@ __global__ void __launch_bounds__(kEntriesBlock) k_entries(Dev d, const int *__r
     __syncthreads();
     int bad = 0;
     for (int k = 0; k < kTile / kEntriesBlock; k++) {
-        const int e = blockIdx.x * kTile + k * kEntriesBlock + threadIdx.x;  // coalesced
-        if (e >= d.nnz) continue;
+        const int e = e0 + blockIdx.x * kTile + k * kEntriesBlock + threadIdx.x;  // coalesced
+        if (e >= e1) continue;
         const int c = cols1[e] - 1, r = rows1[e] - 1;
         int b = 0;
-        if (c < 0 || c >= d.m || r < 0 || r >= c) b |= kErrRange;
-        if (e > 0) {
+        if (c < a || c >= d.m || r < a || r >= c) b |= kErrRange;
+        if (e > e0) {
             const int pc = cols1[e - 1] - 1;
             if (c < pc) b |= kErrOrder;
             else if (c == pc && rows1[e - 1] - 1 >= r) b |= kErrRows;
@@ -405,8 +419,8 @@ __global__ void __launch_bounds__(kEntriesBlock) k_entries(Dev d, const int *__r
     }
 }
 
-__global__ void k_beta(Dev d) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_beta(Dev d, int a) {
+    int i = a + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.m) return;
     int l = d.left[i];
     d.pivdesc[i].w = l;
@@ -414,10 +428,16 @@ __global__ void k_beta(Dev d) {
 }
 
 void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t s) {
-    const int ge = (d.nnz + kTile - 1) / kTile;
-    if (d.m) k_init_cols<<<init_cols_blocks(d.m), kBlock, 0, s>>>(d, rows1, cols1);
-    launch_scan(d, 0, d.off, s);
-    if (d.nnz) k_entries<<<ge, kEntriesBlock, 0, s>>>(d, rows1, cols1);
+    launch_setup_range(d, 0, 0, d.nnz, rows1, cols1, s);
+}
+
+void launch_setup_range(const Dev &d, int a, int e0, int e1, const int *rows1, const int *cols1,
+                        hipStream_t s) {
+    const int ge = (e1 - e0 + kTile - 1) / kTile;
+    if (d.m > a)
+        k_init_cols<<<init_cols_blocks(d.m - a), kBlock, 0, s>>>(d, rows1, cols1, a, e0, e1);
+    launch_scan(d, 0, d.off, s, a, a > 0);
+    if (e1 > e0) k_entries<<<ge, kEntriesBlock, 0, s>>>(d, rows1, cols1, a, e0, e1);
 }
 
 // Pseudo-dimension slots: pseudo-dim = len - 1 (dimensions.cu:61-63). The
@@ -439,8 +459,8 @@ __global__ void k_slotmap(Dev d) {
     if (n > kMaxSlots) atomicOr(&d.ctrl->err, kErrSlots);
 }
 
-__global__ void k_assign_slot(Dev d) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_assign_slot(Dev d, int a) {
+    int j = a + blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= d.m) return;
     int s = d.slotmap[d.len[j]];
     d.slot[j] = s < 0 ? 0 : (s >= kMaxSlots ? kMaxSlots - 1 : s);
@@ -455,10 +475,24 @@ void launch_slots(const Dev &d, hipStream_t s) {
     int gm = (d.m + kBlock - 1) / kBlock;
     int g1 = (d.m + 1 + kBlock - 1) / kBlock;
     k_zero_i<<<g1, kBlock, 0, s>>>(d.slotmap, d.m + 1);
-    k_beta<<<gm, kBlock, 0, s>>>(d);
+    k_beta<<<gm, kBlock, 0, s>>>(d, 0);
     k_presence<<<gm, kBlock, 0, s>>>(d);
     k_slotmap<<<1, 64, 0, s>>>(d);
-    k_assign_slot<<<gm, kBlock, 0, s>>>(d);
+    k_assign_slot<<<gm, kBlock, 0, s>>>(d, 0);
+}
+
+// Columns [a, d.m) added to a running batch: left/beta and the slots from a
+// slot map the host keeps (Ctrl::nslots set with it).
+__global__ void k_set_nslots(Ctrl *c, int n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) c->nslots = n;
+}
+
+void launch_slots_range(const Dev &d, int a, int nslots, hipStream_t s) {
+    const int gm = (d.m - a + kBlock - 1) / kBlock;
+    if (gm <= 0) return;
+    k_beta<<<gm, kBlock, 0, s>>>(d, a);
+    k_assign_slot<<<gm, kBlock, 0, s>>>(d, a);
+    k_set_nslots<<<1, 64, 0, s>>>(d.ctrl, nslots);
 }
 
 // Publishes column j as the pivot of row l.
@@ -471,8 +505,8 @@ __device__ __forceinline__ void publish_pivot(const Dev &d, int l, int j) {
 // Phase 0: mark_pivots_and_clear (pms_functions.cu:33-53)
 // ---------------------------------------------------------------------------
 
-__global__ void k_phase0(Dev d) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_phase0(Dev d, int a) {
+    int j = a + blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= d.m) return;
     int lj = d.low[j];
     if (lj > -1 && lj == d.beta[j]) {
@@ -534,8 +568,29 @@ __global__ void __launch_bounds__(kBlock) k_apply1(Dev d) {
 void launch_phase0(const Dev &d, hipStream_t s) {
     int gm = (d.m + kBlock - 1) / kBlock;
     if (!d.m) return;
-    k_phase0<<<gm, kBlock, 0, s>>>(d);
+    k_phase0<<<gm, kBlock, 0, s>>>(d, 0);
     k_apply1<<<d.ntiles, kBlock, 0, s>>>(d);
+}
+
+// Phase 0 of columns [a, d.m) added to a running batch, its clears and the
+// reset of first[] (the rest of k_apply1: the next iteration builds the
+// worklist).
+__global__ void k_clear_range(Dev d, int a) {
+    const int t = a + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.m) return;
+    if (d.clear[t]) {
+        d.low[t] = -1;
+        d.classes[t] = 1;
+        d.clear[t] = 0;
+    }
+    d.first[t] = kIntMax;
+}
+
+void launch_phase0_range(const Dev &d, int a, hipStream_t s) {
+    const int gm = (d.m - a + kBlock - 1) / kBlock;
+    if (gm <= 0) return;
+    k_phase0<<<gm, kBlock, 0, s>>>(d, a);
+    k_clear_range<<<gm, kBlock, 0, s>>>(d, a);
 }
 
 void launch_apply1(const Dev &d, hipStream_t s) {
@@ -650,6 +705,8 @@ __global__ void __launch_bounds__(kBlock) k_first(Dev d) {
         iter_begin(d.ctrl);
         if (d.its) d.its[2 * (d.ctrl->iter % kIterRing)] = (unsigned long long)wall_clock64();
     }
+    if (blockIdx.x == 0)  // k_tile marks the members whose lows repeat
+        for (int k = threadIdx.x; k < d.nmem; k += kBlock) d.mdup[k] = 0;
     for (int i = threadIdx.x; i < kHash; i += kBlock) {
         hkey[i] = -1;
         hval[i] = kIntMax;
@@ -695,6 +752,17 @@ __device__ __forceinline__ void dup_state(const Dev &d, int j, int lj, int &dup,
     else if (f < j) dup = lj;
 }
 
+__device__ __forceinline__ int segment_of(const int *off, int count, int x) {
+    // largest k with off[k] <= x (off has count+1 ascending entries)
+    int lo = 0, hi = count;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (off[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Per tile of kTile columns: max duplicate low per slot, and the global
 // "some low occurs twice" flag (is_reduced, pms_functions.cu:7-27).
 __global__ void __launch_bounds__(kBlock) k_tile(Dev d) {
@@ -706,6 +774,7 @@ __global__ void __launch_bounds__(kBlock) k_tile(Dev d) {
     __syncthreads();
     const int base = blockIdx.x * kTile + threadIdx.x * kItems;
     int dupseen = 0;
+    int mk = -1;  // member of the last repeating column (running batches)
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
         int j = base + k;
@@ -715,6 +784,10 @@ __global__ void __launch_bounds__(kBlock) k_tile(Dev d) {
             if (dup > -1) {
                 atomicMax(&smax[d.slot[j]], dup);
                 dupseen = 1;
+                if (d.nmem) {
+                    if (mk < 0 || j >= d.mstart[mk + 1]) mk = segment_of(d.mstart, d.nmem, j);
+                    d.mdup[mk] = 1;
+                }
             }
         }
     }
@@ -2286,6 +2359,37 @@ __global__ void k_classes_final(Dev d) {
     }
 }
 
+// A member [a, b) of a running batch whose lows no longer repeat: its final
+// ess (k_final_ess on the range) and output, then its columns inert.
+__global__ void k_final_ess_range(Dev d, int a, int b) {
+    const int t = a + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b) return;
+    const int lt = d.low[t];
+    if (lt > -1) {
+        d.ess[t] = 0;
+        d.ess[lt] = 0;
+    }
+}
+
+__global__ void k_output_retire(Dev d, int a, int b, int *__restrict__ low1,
+                                int *__restrict__ ess) {
+    const int t = a + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b) return;
+    const int lt = d.low[t];
+    low1[t - a] = lt > -1 ? lt - a + 1 : 0;
+    ess[t - a] = d.ess[t];
+    d.low[t] = -1;
+    d.classes[t] = 1;
+    d.arglow[t] = -1;
+}
+
+void launch_retire_range(const Dev &d, int a, int b, int *low1_out, int *ess_out, hipStream_t s) {
+    const int g = (b - a + kBlock - 1) / kBlock;
+    if (g <= 0) return;
+    k_final_ess_range<<<g, kBlock, 0, s>>>(d, a, b);
+    k_output_retire<<<g, kBlock, 0, s>>>(d, a, b, low1_out, ess_out);
+}
+
 void launch_classes_final(const Dev &d, hipStream_t s) {
     if (d.m) k_classes_final<<<(d.m + kBlock - 1) / kBlock, kBlock, 0, s>>>(d);
 }
@@ -2541,16 +2645,6 @@ void launch_compact_copy(Dev &d, hipStream_t s) {
 // [noff[k], noff[k+1]); its input indices are local and 1-based.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ int segment_of(const int *off, int count, int x) {
-    // largest k with off[k] <= x (off has count+1 ascending entries)
-    int lo = 0, hi = count;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (off[mid] <= x) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
 
 __global__ void k_batch_offsets(const int *__restrict__ rows1, const int *__restrict__ cols1,
                                 const int *__restrict__ noff, const int *__restrict__ moff,

@@ -343,6 +343,9 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
     d.nnz = nnz;
     d.ntiles = (m + kTile - 1) / kTile;
     d.slot0 = cfg.slot0;
+    d.nmem = 0;  // not a running batch of the server
+    d.mdup = nullptr;
+    d.mstart = nullptr;
     if (!c.num_cu) {
         hipDeviceProp_t prop;
         OPH_CHECK(hipGetDeviceProperties(&prop, c.device));
@@ -670,6 +673,156 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
             }
     }
     return OPENPH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Running batches (the server): members join between chunks of iterations and
+// leave when their own lows stop repeating.
+// ---------------------------------------------------------------------------
+
+int live_start(Context &c, LiveBatch &lb, const RunCfg &cfg, int cls, int max_members,
+               size_t pool_elems, hipStream_t s) {
+    Dev &d = c.d;
+    if (!c.num_cu) {
+        hipDeviceProp_t prop;
+        OPH_CHECK(hipGetDeviceProperties(&prop, c.device));
+        c.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    }
+    if (!c.h_ctrl) OPH_CHECK(hipHostMalloc((void **)&c.h_ctrl, sizeof(Ctrl), hipHostMallocDefault));
+    if (!c.h_word) OPH_CHECK(hipHostMalloc((void **)&c.h_word, 4 * sizeof(unsigned long long),
+                                           hipHostMallocDefault));
+    OPH_CHECK(grow(&lb.d_mstart, lb.c_mstart, (size_t)max_members + 1));
+    OPH_CHECK(grow(&lb.d_mdup, lb.c_mdup, (size_t)max_members + 1));
+    if (lb.c_hmdup < (size_t)max_members + 1) {
+        if (lb.h_mdup) (void)hipHostFree(lb.h_mdup);
+        lb.h_mdup = nullptr;
+        OPH_CHECK(hipHostMalloc((void **)&lb.h_mdup, sizeof(int) * (max_members + 1),
+                                hipHostMallocDefault));
+        lb.c_hmdup = (size_t)max_members + 1;
+    }
+    d.m = 0;
+    d.nnz = 0;
+    d.ntiles = 0;
+    d.slot0 = cfg.slot0;
+    d.ltrue = nullptr;
+    d.trk_part = d.trk_ring = d.its = nullptr;
+    d.mdup = lb.d_mdup;
+    d.mstart = lb.d_mstart;
+    d.nmem = 0;
+    lb.cls = cls;
+    lb.lanes = cfg.lanes > 1 ? cfg.lanes : (cls == 0 ? 8 : 64);
+    lb.wlds = cfg.lanes > 1 ? cfg.wlds : (cls == 0 ? 80 : 1024);
+    if (!phase2_shape_ok(lb.lanes, lb.wlds)) return OPENPH_EINVAL;
+    lb.grid = 2 * phase2_blocks_per_cu(lb.lanes, lb.wlds) * c.num_cu;
+    lb.long_grid = c.num_cu;
+    d.wide_max = lb.wlds >= 1024 && cfg.wide ? 3 * c.num_cu : 0;
+    lb.mstart.assign(1, 0);
+    lb.slot_of_len.clear();
+    lb.nslots = 0;
+    lb.compactions = 0;
+    lb.lcap = std::min(c.cap_pool, std::min(pool_elems, cfg.pool_max));
+    launch_ctrl_init(d, (unsigned long long)lb.lcap, 1 << 30, s);
+    c.h_ctrl->iter = 0;
+    c.h_ctrl->pool_top = 0;
+    c.h_ctrl->halt = kHaltNone;
+    lb.running = true;
+    return OPENPH_OK;
+}
+
+int live_append(Context &c, LiveBatch &lb, int count, const int *ms, const int *nnzs,
+                const std::vector<int> *lens, const int *rows1, const int *cols1, int nnz,
+                hipStream_t s) {
+    Dev &d = c.d;
+    if (count <= 0) return OPENPH_OK;
+    const int a = d.m;
+    std::vector<int> moff((size_t)count + 1), noff((size_t)count + 1);
+    long long M = a, N = 0;
+    for (int k = 0; k < count; k++) {
+        moff[k] = (int)M;
+        noff[k] = (int)N;
+        M += ms[k];
+        N += nnzs[k];
+    }
+    moff[count] = (int)M;
+    noff[count] = (int)N;
+    if (M > 0x7fffffffLL || N != nnz) return OPENPH_EINVAL;
+    // pseudo-dimension slots: lengths not seen in the batch get new ones
+    int maxl = (int)lb.slot_of_len.size() - 1;
+    for (int k = 0; k < count; k++)
+        for (int L : lens[k]) maxl = std::max(maxl, L);
+    if ((size_t)maxl + 1 > lb.slot_of_len.size()) lb.slot_of_len.resize((size_t)maxl + 1, -1);
+    for (int k = 0; k < count; k++)
+        for (int L : lens[k])
+            if (lb.slot_of_len[L] < 0) lb.slot_of_len[L] = lb.nslots++;
+    if (lb.nslots > kMaxSlots) return OPENPH_EDIMS;
+    OPH_CHECK(grow(&c.b_moff, c.c_bmoff, (size_t)count + 1));
+    OPH_CHECK(grow(&c.b_noff, c.c_bnoff, (size_t)count + 1));
+    OPH_CHECK(grow(&c.b_err, c.c_berr, 1));
+    // the staging below is pageable host memory copied before this call
+    // returns (hipMemcpyAsync from pageable memory completes the copy)
+    OPH_CHECK(hipMemcpyAsync(c.b_moff, moff.data(), sizeof(int) * (count + 1),
+                             hipMemcpyHostToDevice, s));
+    OPH_CHECK(hipMemcpyAsync(c.b_noff, noff.data(), sizeof(int) * (count + 1),
+                             hipMemcpyHostToDevice, s));
+    OPH_CHECK(hipMemcpyAsync(d.slotmap, lb.slot_of_len.data(), sizeof(int) * (maxl + 1),
+                             hipMemcpyHostToDevice, s));
+    launch_batch_offsets(rows1, cols1, c.b_noff, c.b_moff, count, nnz, c.b_rows, c.b_cols,
+                         c.b_err, s);
+    d.m = (int)M;
+    d.ntiles = (int)((M + kTile - 1) / kTile);
+    launch_setup_range(d, a, 0, nnz, c.b_rows, c.b_cols, s);
+    launch_slots_range(d, a, lb.nslots, s);
+    launch_phase0_range(d, a, s);
+    for (int k = 1; k <= count; k++) lb.mstart.push_back(moff[k]);
+    d.nmem = (int)lb.mstart.size() - 1;
+    OPH_CHECK(hipMemcpyAsync(lb.d_mstart, lb.mstart.data(), sizeof(int) * lb.mstart.size(),
+                             hipMemcpyHostToDevice, s));
+    OPH_CHECK(hipStreamSynchronize(s));  // the pageable staging above
+    return OPENPH_OK;
+}
+
+int live_chunk(Context &c, LiveBatch &lb, const RunCfg &cfg, int iters, hipStream_t s) {
+    Dev &d = c.d;
+    const Shape sh{lb.lanes, lb.wlds, lb.grid, lb.long_grid};
+    for (int q = 0; q < iters; q++) {
+        int rc = enqueue_iteration(d, sh, s, nullptr, nullptr, nullptr);
+        if (rc) return rc;
+    }
+    if (d.nmem)
+        OPH_CHECK(hipMemcpyAsync(lb.h_mdup, lb.d_mdup, sizeof(int) * d.nmem,
+                                 hipMemcpyDeviceToHost, s));
+    int rc = read_ctrl(c, s);
+    if (rc) return rc;
+    const Ctrl &h = *c.h_ctrl;
+    if (h.halt == kHaltCompact) {
+        RunCfg cc = cfg;
+        cc.pool_max = std::min(cfg.pool_max, c.cap_pool2 ? std::max(c.cap_pool, c.cap_pool2)
+                                                         : c.cap_pool);
+        if ((rc = compact(c, cc, s, &lb.compactions, lb.lcap))) return rc;
+        launch_set_halt(d, kHaltNone, s);
+        c.h_ctrl->halt = kHaltNone;
+        return OPENPH_OK;
+    }
+    if (h.halt && h.halt != kHaltConverged) return halt_status(h);
+    return OPENPH_OK;
+}
+
+int live_retire(Context &c, LiveBatch &lb, int k, int *low1, int *ess, hipStream_t s) {
+    if (k < 0 || k + 1 >= (int)lb.mstart.size()) return OPENPH_EINVAL;
+    launch_retire_range(c.d, lb.mstart[k], lb.mstart[k + 1], low1, ess, s);
+    return OPENPH_OK;
+}
+
+void live_trim(Context &c, LiveBatch &lb, const std::vector<char> &retired, hipStream_t s) {
+    Dev &d = c.d;
+    int n = (int)lb.mstart.size() - 1;
+    while (n > 0 && retired[n - 1]) n--;
+    if (n == (int)lb.mstart.size() - 1) return;
+    lb.mstart.resize((size_t)n + 1);
+    d.m = lb.mstart[n];
+    d.ntiles = (d.m + kTile - 1) / kTile;
+    d.nmem = n;
+    (void)s;
 }
 
 // The algorithms other than pms run their own order on the device.

@@ -5,6 +5,7 @@ import json
 import os
 import sys
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -135,3 +136,103 @@ def test_server_builds_point_clouds_and_fields_on_the_device():
             assert info.m == b.m
         with pytest.raises(openph_amd.OpenPHError):
             srv.submit_points(np.full((4, 3), np.nan))
+
+
+def test_server_admits_into_a_running_batch_and_releases_members_early():
+    """One worker: a large cubical grid starts a batch; a small grid of the
+    same class submitted while it runs joins that batch at an iteration
+    boundary and leaves it as soon as its own lows stop repeating, before
+    the large one is done. Both equal the oracle."""
+    from openph_amd.server import Server
+    print("server continuous batching", flush=True)
+    big = C.cubical_lower_star(C.smoothed_field(np.random.default_rng(21), 80))
+    small = C.cubical_lower_star(C.smoothed_field(np.random.default_rng(22), 10))
+    with Server(workers=1) as srv:
+        rb, cb = big.coo1()
+        tb = srv.submit_coo(rb, cb, big.m)
+        t0 = time.perf_counter()
+        while srv.stats()["queued"] and time.perf_counter() - t0 < 30:
+            time.sleep(0.0005)
+        ts = srv.submit(small)
+        low_s, ess_s, info_s = srv.wait(ts)
+        low_b, ess_b, info_b = srv.wait(tb)
+    ref_s, ref_b = oracle.pms(small), oracle.pms(big)
+    assert np.array_equal(low_s, ref_s.low) and np.array_equal(ess_s, ref_s.ess)
+    assert np.array_equal(low_b, ref_b.low) and np.array_equal(ess_b, ref_b.ess)
+    assert info_s.worker == info_b.worker == 0
+    assert info_s.start_s > info_b.start_s  # joined the running batch later
+    assert info_s.done_s < info_b.done_s    # and left it first
+    assert info_b.batch_size >= 2
+
+
+def _full_simplex(n):
+    """All faces of the simplex on n vertices (by dimension, then
+    lexicographic): column lengths 0, 2, ..., n."""
+    import itertools
+    cells = [c for k in range(1, n + 1) for c in itertools.combinations(range(n), k)]
+    pos = {c: i for i, c in enumerate(cells)}
+    rows, cols = [], []
+    for j, c in enumerate(cells):
+        if len(c) > 1:
+            f = sorted(pos[c[:i] + c[i + 1:]] for i in range(len(c)))
+            rows += [x + 1 for x in f]
+            cols += [j + 1] * len(f)
+    return np.array(rows, np.int32), np.array(cols, np.int32), len(cells)
+
+
+def _cube_cells(d):
+    """All faces of the d-cube (coordinates 0 and 2 for the ends, 1 for the
+    interval; by dimension, then index): column lengths 0, 2, 4, ..., 2d."""
+    import itertools
+    cells = sorted(itertools.product((0, 1, 2), repeat=d), key=lambda c: (sum(x == 1 for x in c), c))
+    pos = {c: i for i, c in enumerate(cells)}
+    rows, cols = [], []
+    for j, c in enumerate(cells):
+        f = sorted(pos[c[:i] + (e,) + c[i + 1:]] for i, x in enumerate(c) if x == 1 for e in (0, 2))
+        rows += [x + 1 for x in f]
+        cols += [j + 1] * len(f)
+    return np.array(rows, np.int32), np.array(cols, np.int32), len(cells)
+
+
+def test_server_keeps_batches_within_the_pseudo_dimension_limit():
+    """The full simplex on 16 vertices (column lengths 0, 2..16) and the
+    faces of a 9-cube (0, 2, 4, ..., 18): 17 distinct lengths over both,
+    more than a batch can tell apart (OPENPH_MAX_DIMS = 16). They never
+    share a batch, and both succeed. Both are contractible: one essential
+    class each."""
+    from openph_amd.server import Server
+    print("server length union", flush=True)
+    cases = [_full_simplex(16), _cube_cells(9)]
+    with Server(workers=1) as srv:
+        tickets = [srv.submit_coo(*x) for x in cases]
+        res = [srv.wait(t) for t in tickets]
+    for (r, c, m), (low, ess, info) in zip(cases, res):
+        assert info.status == 0 and info.batch_size == 1
+        ref = openph_amd.ph("pms", r, c, m)
+        assert np.array_equal(low + 1, ref[0]) and np.array_equal(ess, ref[1])
+        assert ess.sum() == 1
+
+
+def test_server_ticket_is_waited_for_once():
+    """A second wait on a ticket that another thread is waiting for is
+    refused (OPENPH_EINVAL) instead of reading a request that the first
+    waiter frees."""
+    from openph_amd.server import Server
+    import ctypes
+    print("server double wait", flush=True)
+    b = C.cubical_lower_star(C.smoothed_field(np.random.default_rng(5), 48))
+    with Server(workers=1) as srv:
+        t = srv.submit(b)
+        out = {}
+        th = threading.Thread(target=lambda: out.setdefault("r", srv.wait(t)))
+        th.start()
+        time.sleep(0.002)
+        low = np.empty(b.m, np.int32)
+        ess = np.empty(b.m, np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        from openph_amd.server import _bind
+        rc = _bind().openph_server_wait(srv._h, t, low.ctypes.data_as(ip), ess.ctypes.data_as(ip),
+                                        None)
+        th.join()
+    assert rc == -1
+    assert np.array_equal(out["r"][0], oracle.pms(b).low)
