@@ -499,7 +499,8 @@ def _served_built_stream(seq, workers, seed, what):
                     if dt > 0:
                         time.sleep(dt)
                     kind, x = inputs[i]
-                    tickets[k] = srv.submit_points(x) if kind == "points" else srv.submit_field(x)
+                    tickets[k] = (srv.submit_points(x, borrow=True) if kind == "points"
+                                  else srv.submit_field(x, borrow=True))
 
             th = threading.Thread(target=submitter)
             th.start()
@@ -538,26 +539,41 @@ def _served_stream(pool, seq, workers, seed, what):
     ref = [openph_amd.reduce(b, "pms")[:2] for b in pool]  # warm-up and outputs alone
     cols = sum(pool[i].m for i in seq)
     t0 = time.perf_counter()
+    dev_ms = 0.0
     for i in seq:
-        openph_amd.reduce(pool[i], "pms")
+        dev_ms += openph_amd.reduce(pool[i], "pms")[2].device_ms
     t_plain = time.perf_counter() - t0
+    # an untimed pass first: the server path's first use in the process
+    _served_run(pool, coo, ref, seq, cols, None, workers, np.random.default_rng(seed + 1))
     burst = _served_run(pool, coo, ref, seq, cols, None, workers, np.random.default_rng(seed + 1))
+    copied = _served_run(pool, coo, ref, seq, cols, None, workers, np.random.default_rng(seed + 1),
+                         borrow=False)
     qps = len(seq) / t_plain
     paced = _served_run(pool, coo, ref, seq, cols, qps, workers, np.random.default_rng(seed + 1))
     return {
         "workload": what, "requests": len(seq), "columns": cols,
         "plain_sequential_cols_per_s": cols / t_plain, "plain_sequential_s": t_plain,
+        # the device time of those reductions (set-up to outputs, events on
+        # the drop-in's stream): the rest of plain_sequential_s is host work
+        "plain_device_s": dev_ms / 1e3,
         "served_cols_per_s": burst["served_cols_per_s"],
         "speedup_served_vs_plain": burst["served_cols_per_s"] / (cols / t_plain),
         "latency_ms_p50": paced["latency_ms_p50"], "latency_ms_p99": paced["latency_ms_p99"],
         "latency_qps": qps, "burst": burst, "poisson": paced,
-        "served_equals_alone": burst["served_equals_alone"] and paced["served_equals_alone"],
+        "burst_copied_submit": {k: copied[k] for k in
+                                ("served_cols_per_s", "span_s", "submit_span_s")},
+        "served_equals_alone": (burst["served_equals_alone"] and paced["served_equals_alone"]
+                                and copied["served_equals_alone"]),
     }
 
 
-def _served_run(pool, coo, ref, seq, cols, qps, workers, rng):
+def _served_run(pool, coo, ref, seq, cols, qps, workers, rng, borrow=True):
     """Submit `seq` to a fresh server, all at once (qps None) or with Poisson
-    arrivals at `qps`; wait for every result and compare it with `ref`."""
+    arrivals at `qps`; wait for every result and compare it with `ref`.
+    borrow: zero-copy submission (the worker reads the caller's COO in place
+    and writes the results into arrays handed over at submit, as the
+    reference's mex gateway lends its inputs and outputs); otherwise each
+    request is copied at submit."""
     import threading
 
     from openph_amd.server import Server
@@ -575,7 +591,7 @@ def _served_run(pool, coo, ref, seq, cols, qps, workers, rng):
                 if dt > 0:
                     time.sleep(dt)
                 r, c = coo[i]
-                tickets[k] = srv.submit_coo(r, c, pool[i].m)
+                tickets[k] = srv.submit_coo(r, c, pool[i].m, borrow=borrow)
 
         th = threading.Thread(target=submitter)
         th.start()
@@ -611,6 +627,7 @@ def _served_run(pool, coo, ref, seq, cols, qps, workers, rng):
         "latency_ms_p99": float(np.percentile(lat_ms, 99)),
         "latency_ms_max": float(lat_ms.max()), "by_class": by,
         "batches": st["batches"], "served_equals_alone": same,
+        "submit": "borrowed" if borrow else "copied",
     }
 
 

@@ -259,6 +259,10 @@ typedef struct openph_server_options {
     int max_queue;           /* queued requests before OPENPH_EBUSY
                                 (0 = 65536)                                  */
     openph_options reduce;   /* options of each reduction                    */
+    int64_t stage_bytes;     /* pinned host memory that holds the queued
+                                requests' COO, validated and ready for one
+                                DMA each (0 = 2 GiB, < 0 = none); requests
+                                that do not fit are staged by the workers   */
 } openph_server_options;
 
 typedef struct openph_request_info {
@@ -288,6 +292,37 @@ int openph_server_submit_vr(openph_server *srv, const double *points, int n, int
                             int64_t *ticket);
 int openph_server_submit_cubical(openph_server *srv, const float *field, int s0, int s1, int s2,
                                  int64_t *ticket);
+/* One request of any kind, with optional zero-copy input and caller-owned
+ * outputs (openph_server_submit_request). The fields of the other kinds are
+ * ignored. */
+#define OPENPH_REQ_COO 0      /* rows1/cols1/m/nnz, as openph_server_submit  */
+#define OPENPH_REQ_VR 1       /* points/n/dim, as openph_server_submit_vr    */
+#define OPENPH_REQ_CUBICAL 2  /* field/s0/s1/s2, as openph_server_submit_cubical */
+#define OPENPH_BORROW 1       /* flags: the input arrays are read in place    */
+typedef struct openph_request {
+    int kind;                /* OPENPH_REQ_*                                 */
+    int flags;               /* 0 (inputs copied at submit) or OPENPH_BORROW */
+    const int *rows1, *cols1;
+    int m, nnz;
+    const double *points;
+    int n, dim;
+    const float *field;
+    int s0, s1, s2;
+    int *low1, *ess;         /* NULL, or host buffers of the request's m
+                                columns that the worker writes its outputs
+                                into (both or neither)                       */
+} openph_request;
+/* Queue a request described by `req`; validation, errors and the queue are
+ * those of the matching submit call. With OPENPH_BORROW nothing is copied at
+ * submit: the input arrays are read by the worker when the request joins a
+ * batch, so they must stay valid and unchanged until openph_server_wait has
+ * returned for the ticket (or openph_server_destroy has). A lent COO is
+ * validated after submit has returned, by the server's intake threads: a
+ * malformed one completes with its error code, which wait returns. Output
+ * buffers given in req->low1/ess receive the results before the request is
+ * released and must stay valid for as long; wait may then pass NULL for its
+ * own. */
+int openph_server_submit_request(openph_server *srv, const openph_request *req, int64_t *ticket);
 /* Block until the ticket's reduction is done; low1/ess (host, m entries) get
  * its outputs. A ticket can be waited for once: a second wait, also one that
  * starts while the first blocks, returns OPENPH_EINVAL. */

@@ -211,6 +211,11 @@ struct LiveBatch {
     int compactions = 0;
     int *d_mstart = nullptr, *d_mdup = nullptr, *h_mdup = nullptr;
     size_t c_mstart = 0, c_mdup = 0, c_hmdup = 0;
+    // pinned staging of an append's offsets, slot map and member starts,
+    // reused once the event recorded after their upload has completed
+    int *h_stage = nullptr;
+    size_t c_hstage = 0;
+    hipEvent_t ev_stage = nullptr;
     LiveBatch() = default;
     LiveBatch(const LiveBatch &) = delete;
     LiveBatch &operator=(const LiveBatch &) = delete;
@@ -218,6 +223,8 @@ struct LiveBatch {
         if (d_mstart) (void)hipFree(d_mstart);
         if (d_mdup) (void)hipFree(d_mdup);
         if (h_mdup) (void)hipHostFree(h_mdup);
+        if (h_stage) (void)hipHostFree(h_stage);
+        if (ev_stage) (void)hipEventDestroy(ev_stage);
     }
 };
 // Starts an empty batch of class cls on the workspace, with room for
@@ -228,14 +235,17 @@ int live_start(Context &c, LiveBatch &lb, const RunCfg &cfg, int cls, int max_me
 // lengths of member k's input (their union with the batch's must stay within
 // kMaxSlots), rows1/cols1 their local 1-based COO concatenated on the device
 // (nnz entries). The workspace must have room for the columns and entries.
+// Everything is enqueued on s; nothing waits for the device.
 int live_append(Context &c, LiveBatch &lb, int count, const int *ms, const int *nnzs,
                 const std::vector<int> *lens, const int *rows1, const int *cols1, int nnz,
                 hipStream_t s);
-// Queues `iters` iterations and reads the control block and the members'
-// duplicate flags (lb.h_mdup) back; compacts the pool when asked. Returns an
-// OPENPH_E* code when the reduction failed; c.h_ctrl->halt is
-// kHaltConverged when no low of any member repeats.
-int live_chunk(Context &c, LiveBatch &lb, const RunCfg &cfg, int iters, hipStream_t s);
+// Queues `iters` iterations and the copies of the control block and the
+// members' duplicate flags (lb.h_mdup) to the host, without waiting.
+int live_chunk_launch(Context &c, LiveBatch &lb, int iters, hipStream_t s);
+// Waits for the chunk, compacts the pool when asked. Returns an OPENPH_E*
+// code when the reduction failed; c.h_ctrl->halt is kHaltConverged when no
+// low of any member repeats.
+int live_chunk_finish(Context &c, LiveBatch &lb, const RunCfg &cfg, hipStream_t s);
 // Member k's 1-based local low and ess into low1/ess (device), its columns
 // made inert; trailing retired members' columns are given back.
 int live_retire(Context &c, LiveBatch &lb, int k, int *low1, int *ess, hipStream_t s);

@@ -693,6 +693,11 @@ int live_start(Context &c, LiveBatch &lb, const RunCfg &cfg, int cls, int max_me
                                            hipHostMallocDefault));
     OPH_CHECK(grow(&lb.d_mstart, lb.c_mstart, (size_t)max_members + 1));
     OPH_CHECK(grow(&lb.d_mdup, lb.c_mdup, (size_t)max_members + 1));
+    // an append's offsets, sized for a full batch here: a device free while
+    // the batch runs would wait for every stream of the device
+    OPH_CHECK(grow(&c.b_moff, c.c_bmoff, (size_t)max_members + 1));
+    OPH_CHECK(grow(&c.b_noff, c.c_bnoff, (size_t)max_members + 1));
+    OPH_CHECK(grow(&c.b_err, c.c_berr, 1));
     if (lb.c_hmdup < (size_t)max_members + 1) {
         if (lb.h_mdup) (void)hipHostFree(lb.h_mdup);
         lb.h_mdup = nullptr;
@@ -758,14 +763,31 @@ int live_append(Context &c, LiveBatch &lb, int count, const int *ms, const int *
     OPH_CHECK(grow(&c.b_moff, c.c_bmoff, (size_t)count + 1));
     OPH_CHECK(grow(&c.b_noff, c.c_bnoff, (size_t)count + 1));
     OPH_CHECK(grow(&c.b_err, c.c_berr, 1));
-    // the staging below is pageable host memory copied before this call
-    // returns (hipMemcpyAsync from pageable memory completes the copy)
-    OPH_CHECK(hipMemcpyAsync(c.b_moff, moff.data(), sizeof(int) * (count + 1),
-                             hipMemcpyHostToDevice, s));
-    OPH_CHECK(hipMemcpyAsync(c.b_noff, noff.data(), sizeof(int) * (count + 1),
-                             hipMemcpyHostToDevice, s));
-    OPH_CHECK(hipMemcpyAsync(d.slotmap, lb.slot_of_len.data(), sizeof(int) * (maxl + 1),
-                             hipMemcpyHostToDevice, s));
+    for (int k = 1; k <= count; k++) lb.mstart.push_back(moff[k]);
+    // offsets, slot map and member starts go up from pinned staging, reused
+    // once the previous append's uploads are done
+    const size_t nm = lb.mstart.size(), ns = (size_t)maxl + 1;
+    const size_t need = 2 * ((size_t)count + 1) + ns + nm;
+    if (!lb.ev_stage) OPH_CHECK(hipEventCreateWithFlags(&lb.ev_stage, hipEventDisableTiming));
+    else OPH_CHECK(hipEventSynchronize(lb.ev_stage));
+    if (lb.c_hstage < need) {
+        // pinned allocations are slow and their frees wait for the device:
+        // room for a full batch, grown by doubling
+        const size_t n = std::max(2 * need, 3 * (lb.c_mstart + 1) + 1024);
+        if (lb.h_stage) (void)hipHostFree(lb.h_stage);
+        lb.h_stage = nullptr;
+        lb.c_hstage = 0;
+        OPH_CHECK(hipHostMalloc((void **)&lb.h_stage, sizeof(int) * n, hipHostMallocDefault));
+        lb.c_hstage = n;
+    }
+    int *hm = lb.h_stage, *hn = hm + count + 1, *hs = hn + count + 1, *hk = hs + ns;
+    std::memcpy(hm, moff.data(), sizeof(int) * (count + 1));
+    std::memcpy(hn, noff.data(), sizeof(int) * (count + 1));
+    std::memcpy(hs, lb.slot_of_len.data(), sizeof(int) * ns);
+    std::memcpy(hk, lb.mstart.data(), sizeof(int) * nm);
+    OPH_CHECK(hipMemcpyAsync(c.b_moff, hm, sizeof(int) * (count + 1), hipMemcpyHostToDevice, s));
+    OPH_CHECK(hipMemcpyAsync(c.b_noff, hn, sizeof(int) * (count + 1), hipMemcpyHostToDevice, s));
+    OPH_CHECK(hipMemcpyAsync(d.slotmap, hs, sizeof(int) * ns, hipMemcpyHostToDevice, s));
     launch_batch_offsets(rows1, cols1, c.b_noff, c.b_moff, count, nnz, c.b_rows, c.b_cols,
                          c.b_err, s);
     d.m = (int)M;
@@ -773,15 +795,13 @@ int live_append(Context &c, LiveBatch &lb, int count, const int *ms, const int *
     launch_setup_range(d, a, 0, nnz, c.b_rows, c.b_cols, s);
     launch_slots_range(d, a, lb.nslots, s);
     launch_phase0_range(d, a, s);
-    for (int k = 1; k <= count; k++) lb.mstart.push_back(moff[k]);
-    d.nmem = (int)lb.mstart.size() - 1;
-    OPH_CHECK(hipMemcpyAsync(lb.d_mstart, lb.mstart.data(), sizeof(int) * lb.mstart.size(),
-                             hipMemcpyHostToDevice, s));
-    OPH_CHECK(hipStreamSynchronize(s));  // the pageable staging above
+    d.nmem = (int)nm - 1;
+    OPH_CHECK(hipMemcpyAsync(lb.d_mstart, hk, sizeof(int) * nm, hipMemcpyHostToDevice, s));
+    OPH_CHECK(hipEventRecord(lb.ev_stage, s));
     return OPENPH_OK;
 }
 
-int live_chunk(Context &c, LiveBatch &lb, const RunCfg &cfg, int iters, hipStream_t s) {
+int live_chunk_launch(Context &c, LiveBatch &lb, int iters, hipStream_t s) {
     Dev &d = c.d;
     const Shape sh{lb.lanes, lb.wlds, lb.grid, lb.long_grid};
     for (int q = 0; q < iters; q++) {
@@ -791,14 +811,20 @@ int live_chunk(Context &c, LiveBatch &lb, const RunCfg &cfg, int iters, hipStrea
     if (d.nmem)
         OPH_CHECK(hipMemcpyAsync(lb.h_mdup, lb.d_mdup, sizeof(int) * d.nmem,
                                  hipMemcpyDeviceToHost, s));
-    int rc = read_ctrl(c, s);
-    if (rc) return rc;
+    OPH_CHECK(hipMemcpyAsync(c.h_ctrl, c.d.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+    return OPENPH_OK;
+}
+
+int live_chunk_finish(Context &c, LiveBatch &lb, const RunCfg &cfg, hipStream_t s) {
+    Dev &d = c.d;
+    OPH_CHECK(hipStreamSynchronize(s));
     const Ctrl &h = *c.h_ctrl;
     if (h.halt == kHaltCompact) {
         RunCfg cc = cfg;
         cc.pool_max = std::min(cfg.pool_max, c.cap_pool2 ? std::max(c.cap_pool, c.cap_pool2)
                                                          : c.cap_pool);
-        if ((rc = compact(c, cc, s, &lb.compactions, lb.lcap))) return rc;
+        int rc = compact(c, cc, s, &lb.compactions, lb.lcap);
+        if (rc) return rc;
         launch_set_halt(d, kHaltNone, s);
         c.h_ctrl->halt = kHaltNone;
         return OPENPH_OK;

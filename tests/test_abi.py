@@ -68,8 +68,9 @@ def test_library_has_gfx950_code_object():
 def test_struct_layouts_match_the_header(tmp_path):
     """The ctypes mirrors of the public structs have the C layout (sizes and
     field offsets from a C program compiled against include/openph.h)."""
-    from openph_amd.server import RequestInfo, ServerOptions
+    from openph_amd.server import Request, RequestInfo, ServerOptions
     structs = {
+        "openph_request": Request,
         "openph_options": openph_amd.Options,
         "openph_stats": openph_amd.Stats,
         "openph_tune_row": openph_amd.TuneRow,
@@ -92,6 +93,19 @@ def test_struct_layouts_match_the_header(tmp_path):
         assert int(got[cname]) == ctypes.sizeof(py), cname
         for f, _ in py._fields_:
             assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)
+
+
+def test_submit_request_argument_checks_on_host():
+    """A request with one output buffer but not the other, unknown flags or
+    an unknown kind is refused before the server is looked at."""
+    from openph_amd.server import Request, _bind
+    L = _bind()
+    t = ctypes.c_int64()
+    buf = np.zeros(4, np.int32)
+    for q in (Request(kind=0, low1=buf.ctypes.data), Request(kind=0, flags=6),
+              Request(kind=9)):
+        assert L.openph_server_submit_request(None, ctypes.byref(q), ctypes.byref(t)) == -1
+    assert L.openph_server_submit_request(None, None, ctypes.byref(t)) == -1
 
 
 def test_autotune_argument_checks_on_host():

@@ -236,3 +236,55 @@ def test_server_ticket_is_waited_for_once():
         th.join()
     assert rc == -1
     assert np.array_equal(out["r"][0], oracle.pms(b).low)
+
+
+def test_server_borrowed_requests_equal_copied_ones():
+    """Zero-copy requests (openph_server_submit_request with OPENPH_BORROW
+    and caller-owned outputs) of all three kinds, mixed with copied ones in
+    the same batches and, under a small budget, one large enough to run
+    alone: each equals the oracle."""
+    from openph_amd.server import Server
+    print("server borrowed requests", flush=True)
+    rng = np.random.default_rng(31)
+    items = []
+    for k in range(9):
+        if k % 3 == 2:
+            f = C.smoothed_field(rng, int(rng.integers(6, 14)))
+            items.append(("field", f, C.cubical_lower_star(f)))
+        else:
+            p = C.points_cube(rng, int(rng.integers(5, 30)))
+            items.append(("points" if k % 3 == 0 else "coo", p, C.vr_2skeleton(p)))
+    big = C.cubical_lower_star(C.smoothed_field(np.random.default_rng(32), 40))
+    for budget in (8 << 30, 64 << 20):
+        with Server(workers=2, worker_bytes=budget) as srv:
+            tickets = []
+            for k, (kind, x, b) in enumerate(items):
+                lend = k % 2 == 0
+                if kind == "field":
+                    tickets.append(srv.submit_field(x, borrow=lend))
+                elif kind == "points":
+                    tickets.append(srv.submit_points(x, borrow=lend))
+                else:
+                    r, c = b.coo1()
+                    tickets.append(srv.submit_coo(r, c, b.m, borrow=lend))
+            rb, cb = big.coo1()
+            tbig = srv.submit_coo(rb, cb, big.m, borrow=True)
+            # a lent COO is validated after submit: its error comes at wait,
+            # and the other requests are not affected
+            bad = rb.copy()
+            j = int(np.flatnonzero(cb[1:] == cb[:-1])[0])  # two entries of one column
+            bad[j], bad[j + 1] = bad[j + 1], bad[j]          # rows no longer ascending
+            tbad = srv.submit_coo(bad, cb, big.m, borrow=True)
+            with pytest.raises(openph_amd.OpenPHError) as e:
+                srv.wait(tbad)
+            assert e.value.status == -2  # OPENPH_EUNSORTED
+            for t, (kind, x, b) in zip(tickets, items):
+                low, ess, info = srv.wait(t)
+                ref = oracle.pms(b)
+                assert info.status == 0 and info.m == b.m
+                assert np.array_equal(low, ref.low) and np.array_equal(ess, ref.ess), kind
+            low, ess, info = srv.wait(tbig)
+            ref = oracle.pms(big)
+            assert np.array_equal(low, ref.low) and np.array_equal(ess, ref.ess)
+            if budget < (1 << 30):
+                assert info.batch_size == 1
