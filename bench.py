@@ -119,11 +119,19 @@ def cpu_baseline(b, name: str, budget_s: float = 10.0):
     return out, low
 
 
-def device_leg(b, name, steps, warmup, gold=None):
-    """`steps` full pms reductions of `b` through openph_pms_device, COO in
-    HBM before timing starts, low/ess left in HBM. Queued schedule, so that
-    the library brackets every Phase II launch with HIP events on the stream
-    it runs on. Returns the wall time, per-step stats and the result."""
+def device_leg(b, name, steps, warmup, gold=None, timed_steps=5):
+    """Full pms reductions of `b` through openph_pms_device, COO in HBM
+    before timing starts, low/ess left in HBM, in two passes:
+
+    * `steps` timed reductions with the automatic schedule (iterations
+      queued in chunks, replayed from a cached hipGraph once the size
+      repeats) and no per-launch markers: the rate;
+    * `timed_steps` more on the queued schedule with HIP events on the
+      stream around every Phase II launch (options.time_phase2): that
+      kernel's launch times, for its roofline.
+
+    Returns the wall time of the first pass, the stats of both and whether
+    the result equals `gold`."""
     import torch
 
     import openph_amd
@@ -136,7 +144,6 @@ def device_leg(b, name, steps, warmup, gold=None):
     ess = torch.empty(b.m, dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream(dev)
     opt = openph_amd.Options()
-    opt.schedule = openph_amd.Options.QUEUED
     print(f"bench: {name} m={b.m} nnz={b.nnz}, warmup {warmup}, steps {steps}",
           file=sys.stderr, flush=True)
     for _ in range(warmup):
@@ -153,14 +160,21 @@ def device_leg(b, name, steps, warmup, gold=None):
     if gold is not None:
         ok = bool(np.array_equal(low.cpu().numpy(), gold[0] + 1)
                   and np.array_equal(ess.cpu().numpy(), gold[1]))
-    return wall, stats, ok
+    topt = openph_amd.Options()
+    topt.schedule = openph_amd.Options.QUEUED
+    topt.time_phase2 = 1
+    tstats = [openph_amd.pms_device(dr, dc, b.m, low, ess, stream=stream, options=topt)
+              for _ in range(timed_steps)]
+    torch.cuda.synchronize()
+    return wall, stats, tstats, ok
 
 
 def phase2_roofline(stats, m, kernel):
     """Roofline of the Phase II launches: algorithmic bytes per launch =
     (4 * sum over additions of (|a| + |b| + |a xor b|) + 12 * m * launches)
     / launches (each launch reads low, arglow and classes of every column
-    once), over the average launch time (HIP events, median over steps)."""
+    once), over the average launch time (median over the reductions of
+    `stats`, which were run with options.time_phase2)."""
     st = stats[-1]
     launches = max(1, st.phase2_launches)
     p2_bytes = 4.0 * st.merge_elems + 12.0 * m * launches
@@ -187,11 +201,11 @@ def cubical_leg(steps: int = 8, warmup: int = 2, cpu: bool = True):
     t0 = time.perf_counter()
     b = C.cubical_lower_star(C.smoothed_field(np.random.default_rng(0), 64))
     gen_s = time.perf_counter() - t0
-    wall, stats, _ = device_leg(b, "cubical64", steps, warmup)
-    st = stats[-1]
+    wall, stats, tstats, _ = device_leg(b, "cubical64", steps, warmup)
+    st = tstats[-1]
     dev_ms = statistics.median([s.device_ms for s in stats])
     value = b.m * steps / wall
-    roof = phase2_roofline(stats, b.m, f"Phase II: k_phase2_group<{st.lanes}, {st.group_lds}> "
+    roof = phase2_roofline(tstats, b.m, f"Phase II: k_phase2_group<{st.lanes}, {st.group_lds}> "
                                        "+ k_phase2_long")
     # HBM bytes per Phase II pass from the committed counter runs
     tfile = os.path.join(ROOT, "profiles", "pms_cubical64_traffic.json")
@@ -208,7 +222,8 @@ def cubical_leg(steps: int = 8, warmup: int = 2, cpu: bool = True):
                     "GBps": b_alg(st, b.m, b.nnz) / (dev_ms * 1e-3) / 1e9},
         "pms": {"num_iters": st.num_iters, "additions": st.additions,
                 "merge_elems": st.merge_elems, "phase2_ms": statistics.median(
-                    [s.phase2_ms for s in stats]), "handoffs": st.handoffs,
+                    [s.phase2_ms for s in tstats]), "handoffs": st.handoffs,
+                "schedule": stats[-1].schedule,
                 "compactions": st.compactions, "pool_peak": st.pool_peak},
         "host_generation_s": gen_s,
     }
@@ -328,6 +343,13 @@ def batched_leg(count: int = 512, reps: int = 3, tune: bool = True):
     torch.cuda.synchronize()
     tp = time.perf_counter() - t0
     same = bool(torch.equal(low_b, low_p) and torch.equal(ess_b, ess_p))
+    # one more batched call with every Phase II launch timed, for the roofline
+    topt = (openph_amd.Options.from_buffer_copy(best) if best is not None
+            else openph_amd.Options())
+    topt.time_phase2 = 1
+    st_t = openph_amd.pms_batch_device(ms, nnzs, rows, cols, low_b, ess_b, stream=stream,
+                                       options=topt)
+    torch.cuda.synchronize()
     b_s = statistics.median(tb)
     return {
         "workload": f"{count} VR 2-skeleta, N~U{{20..80}}, cube/sphere/5-GMM, seed 0",
@@ -342,11 +364,13 @@ def batched_leg(count: int = 512, reps: int = 3, tune: bool = True):
                   if tune else None),
         "batched_num_iters": st_b.num_iters, "batched_additions": st_b.additions,
         "batched_merge_elems": st_b.merge_elems,
-        "roofline": dict(phase2_roofline([st_b], M, f"k_phase2_group<{st_b.lanes}, "
-                                                     f"{st_b.group_lds}> + k_phase2_mid"),
+        "roofline": dict(phase2_roofline([st_t], M, f"k_phase2_group<{st_t.lanes}, "
+                                                     f"{st_t.group_lds}> + k_phase2_mid"),
                          timing=("device-clock stamps around each Phase II launch "
-                                 "(graph schedule)" if st_b.schedule == 3 else
-                                 "HIP events around each Phase II launch")),
+                                 "(graph schedule), one more batched call"
+                                 if st_t.schedule == 3 else
+                                 "HIP events around each Phase II launch, one more batched "
+                                 "call")),
         "batched_equals_plain": same, "host_generation_s": gen_s,
     }
 
@@ -657,6 +681,7 @@ def batched_only_leg(count: int = 512, reps: int = 5):
     stream = torch.cuda.current_stream(dev)
     opt = openph_amd.Options()
     opt.schedule = openph_amd.Options.QUEUED
+    opt.time_phase2 = 1
     stats, ts = [], []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -665,7 +690,8 @@ def batched_only_leg(count: int = 512, reps: int = 5):
         torch.cuda.synchronize()
         ts.append(time.perf_counter() - t0)
     st = stats[-1]
-    return {"workload": f"{count} VR 2-skeleta, N~U{{20..80}}, seed 0, default options, queued",
+    return {"workload": f"{count} VR 2-skeleta, N~U{{20..80}}, seed 0, default options, queued, "
+                        "Phase II launches timed",
             "columns": M, "batched_ms": statistics.median(ts) * 1e3, "reps": reps,
             "roofline": phase2_roofline(stats, M, f"k_phase2_group<{st.lanes}, {st.group_lds}>"
                                                   " + k_phase2_mid"),
@@ -722,13 +748,15 @@ def main():
     gold = np.load(os.path.join(ROOT, "tests", "golden", "torus100.npz"))
     out = {}
     if "torus" in legs:
-        wall, stats, ok = device_leg(b, "torus100", args.steps, args.warmup,
-                                     (gold["low"], gold["ess"]))
-        st = stats[-1]
+        wall, stats, tstats, ok = device_leg(b, "torus100", args.steps, args.warmup,
+                                             (gold["low"], gold["ess"]))
+        st = tstats[-1]
         dev_ms = statistics.median([s.device_ms for s in stats])
         kname = (f"k_phase2_group<{st.lanes}, {st.group_lds}>" if st.lanes > 1
                  else "k_phase2_lane")
-        roof = phase2_roofline(stats, m, kname)
+        roof = phase2_roofline(tstats, m, kname)
+        roof["timing"] = (f"HIP events around each Phase II launch: {len(tstats)} queued "
+                          "reductions after the timed steps")
         # HBM bytes per launch of the same kernel from the committed rocprofv3
         # counter runs (FETCH_SIZE, WRITE_SIZE scaled by a plain copy)
         tfile = os.path.join(ROOT, "profiles", "pms_torus100_traffic.json")
@@ -757,7 +785,7 @@ def main():
             "hbm_alg": {"bytes": b_alg(st, m, nnz), "device_ms": dev_ms,
                         "GBps": b_alg(st, m, nnz) / (dev_ms * 1e-3) / 1e9 if dev_ms > 0 else 0.0},
             "pms": {"num_iters": st.num_iters, "additions": st.additions,
-                    "schedule": st.schedule, "host_reads": st.host_reads,
+                    "schedule": stats[-1].schedule, "host_reads": stats[-1].host_reads,
                     "merge_elems": st.merge_elems, "compactions": st.compactions,
                     "pool_peak": st.pool_peak, "additions_per_s": st.additions * args.steps / wall,
                     "parity_vs_golden": ok},

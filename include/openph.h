@@ -121,6 +121,12 @@ typedef struct openph_options {
                                 (schedules 2 and 3; 0 = 3). The first chunk
                                 is as long as the last reduction of a matrix
                                 of the same size took.                       */
+    int time_phase2;         /* 1 = time every Phase II pass into
+                                openph_stats.phase2_ms: HIP events around it
+                                (schedules 1 and 2) or device-clock stamps
+                                (schedule 3). Each timed pass adds two
+                                markers to the stream (about 5 us each), so
+                                throughput runs leave it 0.                  */
 } openph_options;
 
 /* Statistics of one reduction. */
@@ -131,7 +137,8 @@ typedef struct openph_stats {
     int64_t merge_elems;     /* sum over additions of |a| + |b| + |a xor b|  */
     int64_t pool_peak;       /* largest pool fill seen, in int32 entries     */
     float device_ms;         /* device time of the reduction loop            */
-    float phase2_ms;         /* summed device time of the Phase II launches  */
+    float phase2_ms;         /* summed device time of the Phase II launches
+                                (0 unless options.time_phase2)               */
     int phase2_launches;     /* number of Phase II launches                  */
     int m;                   /* columns reduced                              */
     int overflows;           /* Phase II passes cut short by a full pool     */

@@ -57,6 +57,7 @@ class Options(ctypes.Structure):
         ("group_lds", ctypes.c_int),
         ("schedule", ctypes.c_int),
         ("chunk_iters", ctypes.c_int),
+        ("time_phase2", ctypes.c_int),
     ]
 
     # values of ``schedule``

@@ -187,7 +187,7 @@ struct RunCfg {
     int lanes = 0;
     int wlds = 0;
     bool track = false;
-    bool timing = true;
+    bool timing = false;    // Phase II passes timed (openph_options.time_phase2)
     bool no_alloc = false;  // the context's buffers only: see run_pms
     bool wide = true;       // 256-lane teams for short worklists of long columns
 };

@@ -96,6 +96,7 @@ RunCfg cfg_from(const openph_options *opt, int nnz) {
         r.track = opt->track != 0;
         r.schedule = opt->schedule;
         r.chunk = opt->chunk_iters;
+        r.timing = opt->time_phase2 != 0;
         if (opt->slot_elems > 0) r.slot0 = std::min(opt->slot_elems, 1 << 16);
     }
     if (r.pool_elems > r.pool_max) r.pool_elems = r.pool_max;
@@ -462,7 +463,7 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
     // long columns (the 1024-entry shapes): iterations whose worklist fits
     // the GPU's 256-lane teams at once (three per CU) use them
     d.wide_max = sh.wlds >= 1024 && cfg.wide ? 3 * c.num_cu : 0;
-    const bool timed = stats != nullptr && events;
+    const bool timed = stats != nullptr && events && cfg.timing;
 
     if (events) OPH_CHECK(hipEventRecord(c.ev0, s));
     launch_phase0(d, s);
