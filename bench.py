@@ -656,8 +656,48 @@ def _served_run(pool, coo, ref, seq, cols, qps, workers, rng, borrow=True):
 
 
 LEGS = ("torus", "dropin", "cubical", "twist", "builders", "batched", "served", "cpu",
-        "batched_only")
+        "batched_only", "phase_boundaries")
 DEFAULT_LEGS = ("torus", "dropin", "cubical", "twist", "builders", "batched", "served", "cpu")
+
+
+def phase_boundaries_leg(phases: int = 6, reps: int = 200):
+    """What a persistent (cooperative) iteration would trade: the same chain
+    of `phases` grid-wide passes (each reads what the previous wrote across
+    the grid) separated by kernel boundaries, queued or replayed from a
+    hipGraph, or by grid barriers inside one cooperative launch (one arrival
+    counter, or one per XCD group of blocks and then one for the groups), at
+    the column counts of Torus-100 and cubical 64^3. libopenph_probe.so
+    (openph_amd/csrc/phase_probe.hip); every form must leave the same array."""
+    import ctypes
+
+    L = ctypes.CDLL(os.path.join(ROOT, "openph_amd", "libopenph_probe.so"))
+    L.openph_probe_phases.restype = ctypes.c_int
+    L.openph_probe_phases.argtypes = [ctypes.c_int] * 5 + [
+        ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int),
+        ctypes.POINTER(ctypes.c_longlong)]
+    out = {"phases": phases, "reps": reps,
+           "pass": "y[j] = x[j] + x[(j + m/2) mod m], 4 columns per thread"}
+    import torch
+
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    forms = (("queued", 0, 0), ("graph", 1, 0), ("cooperative_flat", 2, 0),
+             ("cooperative_two_level", 3, 0), ("cooperative_flat_one_block_per_cu", 2, cus),
+             ("cooperative_two_level_one_block_per_cu", 3, cus))
+    for label, m in (("torus100_m", 166750), ("cubical64_m", 2048383)):
+        row, sums = {"m": m}, set()
+        for name, mode, cap in forms:
+            ms, blocks, cs = ctypes.c_float(), ctypes.c_int(), ctypes.c_longlong()
+            rc = L.openph_probe_phases(m, phases, reps, mode, cap, ctypes.byref(ms),
+                                       ctypes.byref(blocks), ctypes.byref(cs))
+            if rc != 0:
+                row[name] = {"status": rc}
+                continue
+            sums.add(cs.value)
+            row[name] = {"us_per_chain": ms.value * 1e3 / reps,
+                         "us_per_pass": ms.value * 1e3 / reps / phases, "blocks": blocks.value}
+        row["same_result"] = len(sums) == 1
+        out[label] = row
+    return out
 
 
 def batched_only_leg(count: int = 512, reps: int = 5):
@@ -817,6 +857,9 @@ def main():
     if "batched_only" in legs:
         print("bench: batched call alone", file=sys.stderr, flush=True)
         out["batched_only"] = batched_only_leg()
+    if "phase_boundaries" in legs:
+        print("bench: phase boundaries", file=sys.stderr, flush=True)
+        out["phase_boundaries"] = phase_boundaries_leg()
     if "served" in legs:
         print("bench: served leg", file=sys.stderr, flush=True)
         out["served"] = served_leg()
