@@ -656,8 +656,34 @@ def _served_run(pool, coo, ref, seq, cols, qps, workers, rng, borrow=True):
 
 
 LEGS = ("torus", "dropin", "cubical", "twist", "builders", "batched", "served", "cpu",
-        "batched_only", "phase_boundaries")
+        "batched_only", "phase_boundaries", "iterations")
 DEFAULT_LEGS = ("torus", "dropin", "cubical", "twist", "builders", "batched", "served", "cpu")
+
+
+def iterations_leg(calls: int = 5):
+    """Per-iteration device time of Torus-100 and cubical 64^3 as the
+    reference's time_track reports it: ph('pms') with an all-zeros low_true
+    (the trackers on, as numerics/benchmark_datasets.m:53 calls it), each
+    iteration timed on the device clock from its first kernel to its last
+    (pms.cu:46-103). Median over `calls` calls, per iteration, in us."""
+    import openph_amd
+    from openph_amd import complexes as C
+
+    out = {}
+    for name, b in (("torus100", C.torus100()),
+                    ("cubical64", C.cubical_lower_star(C.smoothed_field(
+                        np.random.default_rng(0), 64)))):
+        r, c = b.coo1()
+        lt = np.zeros(b.m, np.int32)
+        runs = []
+        for _ in range(calls + 1):  # the first call untimed
+            o = openph_amd.ph("pms", r, c, b.m, 8, lt)
+            n = int(o[7][0])
+            runs.append(o[6][1:n].astype(np.float64) * 1e3)
+        t = np.median(np.array(runs[1:]), axis=0)
+        out[name] = {"m": b.m, "num_iters": n, "iteration_us": [round(float(x), 1) for x in t],
+                     "sum_us": float(t.sum())}
+    return out
 
 
 def phase_boundaries_leg(phases: int = 6, reps: int = 200):
@@ -857,6 +883,9 @@ def main():
     if "batched_only" in legs:
         print("bench: batched call alone", file=sys.stderr, flush=True)
         out["batched_only"] = batched_only_leg()
+    if "iterations" in legs:
+        print("bench: per-iteration times", file=sys.stderr, flush=True)
+        out["iterations"] = iterations_leg()
     if "phase_boundaries" in legs:
         print("bench: phase boundaries", file=sys.stderr, flush=True)
         out["phase_boundaries"] = phase_boundaries_leg()
