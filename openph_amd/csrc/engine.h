@@ -126,6 +126,8 @@ void launch_pool_cap(const Dev &d, unsigned long long requested, unsigned long l
                      hipStream_t s);
 // COO -> per-column state and pool layout (slot-granular segments).
 void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t s);
+// Malformed input found by the set-up: all columns made empty, the loop halted.
+void launch_setup_guard(const Dev &d, hipStream_t s);
 void launch_slots(const Dev &d, hipStream_t s);
 void launch_phase0(const Dev &d, hipStream_t s);
 // iter + 1 and the per-iteration flags reset, unless halted. The pms loop
@@ -191,7 +193,8 @@ void launch_ph_row(const Dev &d, const RowLists &r, bool all_dims, unsigned long
                    unsigned long long *start, hipStream_t s);
 // ess = 1 except at the lows (ph_row_parallel's estimate).
 void launch_ess_at_lows(const Dev &d, hipStream_t s);
-void launch_final(const Dev &d, int *low1_out, int *ess_out, hipStream_t s);
+// gated: only if the loop has converged (queued before the host knows)
+void launch_final(const Dev &d, int *low1_out, int *ess_out, hipStream_t s, bool gated = false);
 // 1-based low and ess out, with no final ess pass.
 void launch_output(const Dev &d, int *low1_out, int *ess_out, hipStream_t s);
 // ess_hat_final (kernels.cu:103-111) alone, and update_classes_final

@@ -102,6 +102,7 @@ struct Context {
     GraphEntry graphs[4];
     unsigned long long graph_clock = 0;
     int last_m = -1, last_nnz = -1, last_iters = 0;
+    int last_lanes = 0, last_wlds = 0;  // the Phase II shape the size picked
     unsigned long long *d_stamps = nullptr, *h_stamps = nullptr;  // graph timing
     int wall_khz = 0;                                            // device clock rate
     // trackers: 0-based low_true on the device; per-iteration trackers and

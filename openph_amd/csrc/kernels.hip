@@ -431,6 +431,29 @@ void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t 
     launch_setup_range(d, 0, 0, d.nnz, rows1, cols1, s);
 }
 
+// After the set-up, for a host that does not wait for it: when the set-up
+// found the input malformed, every column is made empty (length 0, no low)
+// and the loop halted, so that the kernels queued behind it, which index by
+// column lengths and lows, touch nothing out of range. (Lows are in range
+// already: k_init_cols clamps them.) The host reports the error at its next
+// read of the control block.
+__global__ void k_setup_guard(Dev d) {
+    if (!d.ctrl->err) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j == 0) {
+        d.ctrl->maxlen = 0;
+        d.ctrl->halt = kHaltError;
+    }
+    if (j < d.m) {
+        d.len[j] = 0;
+        d.low[j] = -1;
+    }
+}
+
+void launch_setup_guard(const Dev &d, hipStream_t s) {
+    k_setup_guard<<<d.m > 0 ? (d.m + kBlock - 1) / kBlock : 1, kBlock, 0, s>>>(d);
+}
+
 void launch_setup_range(const Dev &d, int a, int e0, int e1, const int *rows1, const int *cols1,
                         hipStream_t s) {
     const int ge = (e1 - e0 + kTile - 1) / kTile;
@@ -2312,7 +2335,10 @@ void launch_apply2(const Dev &d, hipStream_t s) {
 // (openph.cu:195-199).
 // ---------------------------------------------------------------------------
 
-__global__ void k_final_ess(Dev d) {
+// gate: only when the loop has converged (queued behind a chunk of
+// iterations before the host knows whether the chunk converged)
+__global__ void k_final_ess(Dev d, int gate) {
+    if (gate && d.ctrl->halt != kHaltConverged) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.m) return;
     int lt = d.low[t];
@@ -2322,26 +2348,27 @@ __global__ void k_final_ess(Dev d) {
     }
 }
 
-__global__ void k_output(Dev d, int *__restrict__ low1, int *__restrict__ ess) {
+__global__ void k_output(Dev d, int *__restrict__ low1, int *__restrict__ ess, int gate) {
+    if (gate && d.ctrl->halt != kHaltConverged) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.m) return;
     low1[t] = d.low[t] + 1;
     ess[t] = d.ess[t];
 }
 
-void launch_final(const Dev &d, int *low1_out, int *ess_out, hipStream_t s) {
+void launch_final(const Dev &d, int *low1_out, int *ess_out, hipStream_t s, bool gated) {
     int gm = (d.m + kBlock - 1) / kBlock;
     if (!d.m) return;
-    k_final_ess<<<gm, kBlock, 0, s>>>(d);
-    k_output<<<gm, kBlock, 0, s>>>(d, low1_out, ess_out);
+    k_final_ess<<<gm, kBlock, 0, s>>>(d, gated ? 1 : 0);
+    k_output<<<gm, kBlock, 0, s>>>(d, low1_out, ess_out, gated ? 1 : 0);
 }
 
 void launch_output(const Dev &d, int *low1_out, int *ess_out, hipStream_t s) {
-    if (d.m) k_output<<<(d.m + kBlock - 1) / kBlock, kBlock, 0, s>>>(d, low1_out, ess_out);
+    if (d.m) k_output<<<(d.m + kBlock - 1) / kBlock, kBlock, 0, s>>>(d, low1_out, ess_out, 0);
 }
 
 void launch_final_ess(const Dev &d, hipStream_t s) {
-    if (d.m) k_final_ess<<<(d.m + kBlock - 1) / kBlock, kBlock, 0, s>>>(d);
+    if (d.m) k_final_ess<<<(d.m + kBlock - 1) / kBlock, kBlock, 0, s>>>(d, 0);
 }
 
 // update_classes_final (kernels.cu:79-91). The stores cannot conflict: a

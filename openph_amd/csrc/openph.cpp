@@ -396,24 +396,37 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
     // logical pool size: the requested one, even when the context holds a
     // larger buffer from an earlier call
     size_t lcap = std::min(c.cap_pool, pool_want);
-    launch_ctrl_init(d, (unsigned long long)lcap, cfg.max_iters, s);
-    launch_setup(d, rows1, cols1, s);
-    if (cfg.pool_elems) launch_pool_cap(d, cfg.pool_elems, lcap, s);
-    rc = read_ctrl(c, s);  // also completes the low_true upload from pageable memory
-    if (rc) return rc;
-    if ((rc = err_from_bits(c.h_ctrl->err))) return rc;
-    lcap = c.h_ctrl->pool_cap;
-    const int maxlen = c.h_ctrl->maxlen;
     // the schedule: graph replay when the last reduction on this workspace
     // had the same size (its graphs are likely cached: a capture costs more
     // than it saves once), queued chunks otherwise. The trackers and the
     // iteration times (time_track) of the queued and graph schedules are
     // kept per iteration on the device and read once per chunk; the sync
     // schedule reads them after every iteration.
+    const bool same_size = c.last_m == m && c.last_nnz == nnz;
     int sched = cfg.schedule;
-    if (sched == kSchedAuto)
-        sched = (c.last_m == m && c.last_nnz == nnz) ? kSchedGraph : kSchedQueued;
+    if (sched == kSchedAuto) sched = same_size ? kSchedGraph : kSchedQueued;
     if (cfg.no_alloc && sched == kSchedGraph) sched = kSchedQueued;
+    // Without trackers the chunked schedules read the device once per
+    // chunk and no more: the set-up is not waited for when the Phase II
+    // shape is known without it (given, or the one this size took last
+    // time: any shape gives the same result), a guard kernel making
+    // malformed input inert instead; and the outputs are queued behind each
+    // chunk, gated on convergence, so the chunk's read also finds them done.
+    const bool one_read = !tracking && sched != kSchedSync;
+    const bool ahead = one_read && (cfg.lanes != 0 || (same_size && c.last_lanes != 0));
+    launch_ctrl_init(d, (unsigned long long)lcap, cfg.max_iters, s);
+    launch_setup(d, rows1, cols1, s);
+    if (cfg.pool_elems) launch_pool_cap(d, cfg.pool_elems, lcap, s);
+    int maxlen = 0;
+    if (ahead) {
+        launch_setup_guard(d, s);  // the error, and the pool's size, come with the first read
+    } else {
+        rc = read_ctrl(c, s);  // also completes the low_true upload from pageable memory
+        if (rc) return rc;
+        if ((rc = err_from_bits(c.h_ctrl->err))) return rc;
+        lcap = c.h_ctrl->pool_cap;
+        maxlen = c.h_ctrl->maxlen;
+    }
     const bool iter_times = tr && tr->time_track;
     const bool ring = sched != kSchedSync && (tracking || iter_times);
     d.ltrue = nullptr;
@@ -450,9 +463,14 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
     // pivot buffer: 22 KB per block, so seven blocks share a CU's 160 KB),
     // longer ones (cubical complexes) a full wave with 1024-entry buffers.
     Shape sh{cfg.lanes, cfg.wlds, 0, 0};
-    if (!sh.lanes) {
+    if (!sh.lanes && ahead) {
+        sh.lanes = c.last_lanes;
+        sh.wlds = c.last_wlds;
+    } else if (!sh.lanes) {
         sh.lanes = maxlen <= 4 ? 8 : 64;
         sh.wlds = maxlen <= 4 ? 80 : 1024;
+        c.last_lanes = sh.lanes;
+        c.last_wlds = sh.wlds;
     }
     if (sh.lanes > 1) {
         // twice the resident blocks: a block whose columns are done makes
@@ -560,6 +578,10 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
                                                 nullptr)))
                         return rc;
             }
+            if (one_read) {
+                launch_final(d, low1, ess, s, true);
+                if (events) OPH_CHECK(hipEventRecord(c.ev1, s));
+            }
             if (d.trk_ring)
                 OPH_CHECK(hipMemcpyAsync(c.h_ring, d.trk_ring,
                                          4 * kIterRing * sizeof(unsigned long long),
@@ -569,6 +591,7 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
                                          hipMemcpyDeviceToHost, s));
             if ((rc = read_ctrl(c, s))) return rc;
             reads++;
+            if (ahead && iter0 == 0) lcap = h.pool_cap;  // set by the set-up
             // iterations of this chunk that reached Phase II: all that began,
             // but the one that found convergence (or the cap) at its start
             const int begun = h.iter - iter0;
@@ -631,9 +654,11 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
         launch_classes_final(d, s);
         launch_track(d, c.t_low, s);
     }
-    launch_final(d, low1, ess, s);
-    if (events) OPH_CHECK(hipEventRecord(c.ev1, s));
-    if ((rc = read_ctrl(c, s))) return rc;
+    if (!one_read) {
+        launch_final(d, low1, ess, s);
+        if (events) OPH_CHECK(hipEventRecord(c.ev1, s));
+        if ((rc = read_ctrl(c, s))) return rc;
+    }
     if (tracking) {
         fill_track(tr, num_iters - 1, m, ess_true, h.trk_l1, (unsigned long long)h.trk_linf,
                    (unsigned long long)h.trk_unreduced, (unsigned long long)h.trk_ess);

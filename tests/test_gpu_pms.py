@@ -263,6 +263,50 @@ def test_malformed_input_is_rejected(kind):
     assert out[1].tolist() == [1, 0, 0, 0, 0, 0, 0]
 
 
+@pytest.mark.parametrize("kind", ["row_ge_col", "unsorted_rows", "out_of_range"])
+def test_same_size_malformed_input_is_rejected_without_a_setup_read(kind):
+    """A reduction of the size the workspace reduced last does not wait for
+    its set-up (the Phase II shape is the last one's); a malformed matrix of
+    that size must still be refused with its code, through a guard kernel
+    that makes it inert, with the outputs left as given, and the valid
+    matrix reduced next must be exact."""
+    import torch
+    say("same-size malformed", kind)
+    b = _gen("vr40")
+    f = np.load(os.path.join(GOLD, "vr40.npz"))
+    r, c = b.coo1()
+    low0, ess0, _ = openph_amd.reduce(b, "pms")  # the workspace's last size
+    assert np.array_equal(low0, f["low"])
+    rr = r.copy()
+    if kind == "row_ge_col":
+        rr[7] = c[7]
+        want = -1
+    elif kind == "unsorted_rows":
+        j = int(np.flatnonzero(c[1:] == c[:-1])[10])
+        rr[j], rr[j + 1] = rr[j + 1], rr[j]
+        want = -2
+    else:
+        rr[-1] = b.m + 5
+        want = -1
+    with pytest.raises(openph_amd.OpenPHError) as e:
+        openph_amd.ph("pms", rr, c, b.m, 1, None)
+    assert e.value.status == want
+    # the device entry, twice at this size: outputs keep the caller's values
+    dev = torch.device("cuda:0")
+    dr, dc = torch.from_numpy(rr).to(dev), torch.from_numpy(c).to(dev)
+    for _ in range(2):
+        low = torch.full((b.m,), -7, dtype=torch.int32, device=dev)
+        ess = torch.full((b.m,), -7, dtype=torch.int32, device=dev)
+        with pytest.raises(openph_amd.OpenPHError) as e:
+            openph_amd.pms_device(dr, dc, b.m, low, ess)
+        assert e.value.status == want
+        torch.cuda.synchronize()
+        assert int(low.min()) == -7 == int(low.max()) and int(ess.min()) == -7
+    low1, ess1, st = openph_amd.reduce(b, "pms")
+    assert np.array_equal(low1, f["low"]) and np.array_equal(ess1, f["ess"])
+    assert st.host_reads == 1
+
+
 def test_dropin_openph_matches_openph_m_fields():
     say("openph()")
     b = C.vr40()
