@@ -238,6 +238,50 @@ def cubical_leg(steps: int = 8, warmup: int = 2, cpu: bool = True):
     return out
 
 
+def vr40_leg(steps: int = 50, warmup: int = 5, cpu: bool = True):
+    """Plain pms on VR-40 (40 points i.i.d. U[0,1]^3, seed 0; m = 10,700),
+    the reference's CPU-sized configuration: column reductions/s with the
+    COO in HBM and host to host (openph_v2: host COO in, host low/ess out),
+    its Phase II roofline, B_alg over device time, and the CPU twist on the
+    same matrix."""
+    import openph_amd
+    from openph_amd import complexes as C
+
+    b = C.vr40()
+    gold = np.load(os.path.join(ROOT, "tests", "golden", "vr40.npz"))
+    wall, stats, tstats, ok = device_leg(b, "vr40", steps, warmup, (gold["low"], gold["ess"]))
+    st = tstats[-1]
+    dev_ms = statistics.median([s.device_ms for s in stats])
+    value = b.m * steps / wall
+    kname = (f"k_phase2_group<{st.lanes}, {st.group_lds}>" if st.lanes > 1 else "k_phase2_lane")
+    openph_amd.reduce(b, "pms")  # host-to-host warm-up
+    ts = []
+    for _ in range(20):
+        t0 = time.perf_counter()
+        openph_amd.reduce(b, "pms")
+        ts.append(time.perf_counter() - t0)
+    host_s = statistics.median(ts)
+    out = {
+        "workload": "pms_vr40_cube", "m": b.m, "nnz": b.nnz,
+        "points": "40 i.i.d. U[0,1]^3, seed 0",
+        "cols_per_s": value, "ms_per_step": wall / steps * 1e3, "steps": steps,
+        "host_to_host_cols_per_s": b.m / host_s, "host_to_host_ms": host_s * 1e3,
+        "roofline": phase2_roofline(tstats, b.m, kname),
+        "hbm_alg": {"bytes": b_alg(st, b.m, b.nnz), "device_ms": dev_ms,
+                    "GBps": b_alg(st, b.m, b.nnz) / (dev_ms * 1e-3) / 1e9 if dev_ms > 0 else 0.0},
+        "pms": {"num_iters": st.num_iters, "additions": st.additions,
+                "merge_elems": st.merge_elems, "schedule": stats[-1].schedule,
+                "parity_vs_golden": ok},
+    }
+    if cpu:
+        print("bench: vr40 cpu baseline", file=sys.stderr, flush=True)
+        cb, cpu_low = cpu_baseline(b, "VR-40", budget_s=3.0)
+        cb["low_sha_match"] = bool(np.array_equal(cpu_low, gold["low"]))
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = value / cb["value"]
+    return out
+
+
 def dropin_leg(b, reps: int = 20):
     """The drop-in as the reference's callers drive it: the C symbol
     openph() (src/cuda/openph.cu:29-35) host to host on Torus-100 with an
@@ -655,9 +699,10 @@ def _served_run(pool, coo, ref, seq, cols, qps, workers, rng, borrow=True):
     }
 
 
-LEGS = ("torus", "dropin", "cubical", "twist", "builders", "batched", "served", "cpu",
+LEGS = ("torus", "vr40", "dropin", "cubical", "twist", "builders", "batched", "served", "cpu",
         "batched_only", "phase_boundaries", "iterations")
-DEFAULT_LEGS = ("torus", "dropin", "cubical", "twist", "builders", "batched", "served", "cpu")
+DEFAULT_LEGS = ("torus", "vr40", "dropin", "cubical", "twist", "builders", "batched", "served",
+                "cpu")
 
 
 def iterations_leg(calls: int = 5):
@@ -866,6 +911,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    if "vr40" in legs:
+        out["vr40"] = vr40_leg(cpu="cpu" in legs)
     if "dropin" in legs:
         print("bench: drop-in with trackers", file=sys.stderr, flush=True)
         out["dropin"] = dropin_leg(b)
