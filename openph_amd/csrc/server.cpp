@@ -900,8 +900,10 @@ static int enqueue(openph_server *srv, std::unique_ptr<Request> r, int64_t *tick
     if (validated) classify(srv, r.get());
     {
         std::lock_guard<std::mutex> g(srv->mu);
-        if ((int)(srv->queue.size() + srv->intake.size()) >= srv->opt.max_queue)
+        if ((int)(srv->queue.size() + srv->intake.size()) >= srv->opt.max_queue) {
+            if (r->arena_off != Arena::kNone) srv->arena.release(r->arena_off, r->arena_bytes);
             return OPENPH_EBUSY;
+        }
         r->id = srv->next_id++;
         r->t_submit = now_s();
         *ticket = r->id;

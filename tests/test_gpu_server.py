@@ -288,3 +288,29 @@ def test_server_borrowed_requests_equal_copied_ones():
             assert np.array_equal(low, ref.low) and np.array_equal(ess, ref.ess)
             if budget < (1 << 30):
                 assert info.batch_size == 1
+
+
+def test_server_full_queue_refuses_and_keeps_serving():
+    """A queue of two: submits beyond it are refused with OPENPH_EBUSY (copied
+    and lent alike, with a small pinned arena that the refused requests must
+    not keep), and every accepted request still equals the oracle."""
+    from openph_amd.server import Server
+    print("server full queue", flush=True)
+    b = C.cubical_lower_star(C.smoothed_field(np.random.default_rng(41), 24))
+    r, c = b.coo1()
+    ref = oracle.pms(b)
+    busy = 0
+    with Server(workers=1, max_queue=2, stage_bytes=6 << 20) as srv:
+        for lend in (False, True):
+            tickets = []
+            for _ in range(12):
+                try:
+                    tickets.append(srv.submit_coo(r, c, b.m, borrow=lend))
+                except openph_amd.OpenPHError as e:
+                    assert e.status == -8  # OPENPH_EBUSY
+                    busy += 1
+            for t in tickets:
+                low, ess, info = srv.wait(t)
+                assert info.status == 0
+                assert np.array_equal(low, ref.low) and np.array_equal(ess, ref.ess)
+    assert busy > 0
