@@ -136,7 +136,8 @@ typedef struct openph_stats {
     int64_t additions;       /* column additions performed                   */
     int64_t merge_elems;     /* sum over additions of |a| + |b| + |a xor b|  */
     int64_t pool_peak;       /* largest pool fill seen, in int32 entries     */
-    float device_ms;         /* device time of the reduction loop            */
+    float device_ms;         /* device time of the reduction: set-up, loop
+                                and outputs                                  */
     float phase2_ms;         /* summed device time of the Phase II launches
                                 (0 unless options.time_phase2)               */
     int phase2_launches;     /* number of Phase II launches                  */

@@ -63,6 +63,13 @@ struct GraphEntry {
     Dev key;
     int lanes = 0, wlds = 0, grid = 0, long_grid = 0, iters = 0;
     bool timed = false;
+    // a whole reduction: the set-up and Phase 0 before the iterations and
+    // the gated outputs after them, for these inputs, outputs and pool
+    bool whole = false;
+    const int *rows1 = nullptr, *cols1 = nullptr;
+    int *low1 = nullptr, *ess = nullptr;
+    size_t lcap = 0, pool_elems = 0;
+    int max_iters = 0;
     unsigned long long used = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;

@@ -216,18 +216,47 @@ static void drop_graph(GraphEntry &g) {
     g = GraphEntry();
 }
 
-// The cached graph of `iters` iterations for the current buffers and shape,
-// captured on the context's own stream (the caller's may be the null
-// stream, which cannot be captured) when there is none. A timed graph
-// brackets Phase II with device-clock stamps: event records inside a
-// capture are refused by this runtime once PyTorch has initialised it.
-static int graph_for(Context &c, const Shape &sh, int iters, bool timed, GraphEntry **out) {
+// What a whole-reduction graph adds around its iterations: the set-up of
+// rows1/cols1 (ctrl init, COO -> pool, validation guard, slots, Phase 0)
+// and the outputs into low1/ess, gated on convergence.
+struct WholeArgs {
+    const int *rows1, *cols1;
+    int *low1, *ess;
+    size_t lcap, pool_elems;
+    int max_iters;
+};
+
+// The set-up of a reduction whose Phase II shape is known without waiting
+// for it: a malformed input is made inert by the guard and reported by the
+// first read of the control block.
+static void enqueue_setup(const Dev &d, const WholeArgs &w, hipStream_t s) {
+    launch_ctrl_init(d, (unsigned long long)w.lcap, w.max_iters, s);
+    launch_setup(d, w.rows1, w.cols1, s);
+    if (w.pool_elems) launch_pool_cap(d, w.pool_elems, w.lcap, s);
+    launch_setup_guard(d, s);
+    launch_slots(d, s);
+    launch_phase0(d, s);
+}
+
+// The cached graph of `iters` iterations for the current buffers and shape
+// (with `whole`, of the whole reduction around them), captured on the
+// context's own stream (the caller's may be the null stream, which cannot
+// be captured) when there is none. A timed graph brackets Phase II with
+// device-clock stamps: event records inside a capture are refused by this
+// runtime once PyTorch has initialised it.
+static int graph_for(Context &c, const Shape &sh, int iters, bool timed, const WholeArgs *whole,
+                     GraphEntry **out) {
     const Dev &d = c.d;
     GraphEntry *hit = nullptr, *victim = &c.graphs[0];
     for (GraphEntry &g : c.graphs) {
+        const bool same_whole =
+            whole ? g.whole && g.rows1 == whole->rows1 && g.cols1 == whole->cols1 &&
+                        g.low1 == whole->low1 && g.ess == whole->ess && g.lcap == whole->lcap &&
+                        g.pool_elems == whole->pool_elems && g.max_iters == whole->max_iters
+                  : !g.whole;
         if (g.valid && std::memcmp(&g.key, &d, sizeof(Dev)) == 0 && g.lanes == sh.lanes &&
             g.wlds == sh.wlds && g.grid == sh.grid && g.long_grid == sh.long_grid &&
-            g.iters == iters && g.timed == timed) {
+            g.iters == iters && g.timed == timed && same_whole) {
             hit = &g;
             break;
         }
@@ -251,9 +280,11 @@ static int graph_for(Context &c, const Shape &sh, int iters, bool timed, GraphEn
             return OPENPH_EDEVICE;
         }
         int rc = OPENPH_OK;
+        if (whole) enqueue_setup(d, *whole, c.stream);
         for (int k = 0; k < iters && !rc; k++)
             rc = enqueue_iteration(d, sh, c.stream, nullptr, nullptr,
                                    timed ? c.d_stamps : nullptr);
+        if (whole && !rc) launch_final(d, whole->low1, whole->ess, c.stream, true);
         if (rc) debug_note("capture of one iteration", last_hip_error());
         e = hipStreamEndCapture(c.stream, &graph);
         if (rc || e != hipSuccess) {
@@ -278,6 +309,16 @@ static int graph_for(Context &c, const Shape &sh, int iters, bool timed, GraphEn
         victim->long_grid = sh.long_grid;
         victim->iters = iters;
         victim->timed = timed;
+        victim->whole = whole != nullptr;
+        if (whole) {
+            victim->rows1 = whole->rows1;
+            victim->cols1 = whole->cols1;
+            victim->low1 = whole->low1;
+            victim->ess = whole->ess;
+            victim->lcap = whole->lcap;
+            victim->pool_elems = whole->pool_elems;
+            victim->max_iters = whole->max_iters;
+        }
         victim->graph = graph;
         victim->exec = exec;
         hit = victim;
@@ -414,12 +455,22 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
     // chunk, gated on convergence, so the chunk's read also finds them done.
     const bool one_read = !tracking && sched != kSchedSync;
     const bool ahead = one_read && (cfg.lanes != 0 || (same_size && c.last_lanes != 0));
-    launch_ctrl_init(d, (unsigned long long)lcap, cfg.max_iters, s);
-    launch_setup(d, rows1, cols1, s);
-    if (cfg.pool_elems) launch_pool_cap(d, cfg.pool_elems, lcap, s);
+    const bool iter_times = tr && tr->time_track;
+    // A repeated reduction replays one graph from the set-up to the gated
+    // outputs: launched one by one, the set-up's kernels leave the device
+    // waiting on the host between them.
+    const bool whole = sched == kSchedGraph && ahead && !iter_times;
+    const WholeArgs wargs{rows1, cols1, low1, ess, lcap, cfg.pool_elems, cfg.max_iters};
+    if (events) OPH_CHECK(hipEventRecord(c.ev0, s));
+    if (!whole) {
+        launch_ctrl_init(d, (unsigned long long)lcap, cfg.max_iters, s);
+        launch_setup(d, rows1, cols1, s);
+        if (cfg.pool_elems) launch_pool_cap(d, cfg.pool_elems, lcap, s);
+    }
     int maxlen = 0;
     if (ahead) {
-        launch_setup_guard(d, s);  // the error, and the pool's size, come with the first read
+        // the error, and the pool's size, come with the first read
+        if (!whole) launch_setup_guard(d, s);
     } else {
         rc = read_ctrl(c, s);  // also completes the low_true upload from pageable memory
         if (rc) return rc;
@@ -427,7 +478,6 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
         lcap = c.h_ctrl->pool_cap;
         maxlen = c.h_ctrl->maxlen;
     }
-    const bool iter_times = tr && tr->time_track;
     const bool ring = sched != kSchedSync && (tracking || iter_times);
     d.ltrue = nullptr;
     d.trk_part = d.trk_ring = d.its = nullptr;
@@ -457,7 +507,7 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
         if (ring) launch_track_iter(d, s);  // the state before Phase 0 (iter 0)
         else if ((rc = track_state(c, tr, 0, s, m, ess_true))) return rc;
     }
-    launch_slots(d, s);
+    if (!whole) launch_slots(d, s);
     // Phase II lanes per column: short input columns (simplicial complexes
     // of small dimension) take 8 lanes with 80-entry LDS buffers (a 16-entry
     // pivot buffer: 22 KB per block, so seven blocks share a CU's 160 KB),
@@ -483,8 +533,7 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
     d.wide_max = sh.wlds >= 1024 && cfg.wide ? 3 * c.num_cu : 0;
     const bool timed = stats != nullptr && events && cfg.timing;
 
-    if (events) OPH_CHECK(hipEventRecord(c.ev0, s));
-    launch_phase0(d, s);
+    if (!whole) launch_phase0(d, s);
     // iteration i is timed from evs[2i] to evs[2i+1] (tic/toc around the
     // loop body, pms.cu:46-103; the trackers are outside)
     std::vector<hipEvent_t> &evs = c.iter_events;
@@ -559,11 +608,14 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
             k = std::min(k, kIterRing / 2);
             if (d.its) launch_clear_stamps(d, s);
             GraphEntry *ge = nullptr;
-            if (sched == kSchedGraph && graph_for(c, sh, k, timed, &ge) != OPENPH_OK) {
+            const bool first_whole = whole && iter0 == 0;
+            if (sched == kSchedGraph &&
+                graph_for(c, sh, k, timed, first_whole ? &wargs : nullptr, &ge) != OPENPH_OK) {
                 // capture or instantiation refused: queue the same launches
                 (void)hipGetLastError();
                 sched = kSchedQueued;
             }
+            if (first_whole && !ge) enqueue_setup(d, wargs, s);
             if (ge) {
                 OPH_CHECK(hipGraphLaunch(ge->exec, s));
                 if (timed)
@@ -579,7 +631,7 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
                         return rc;
             }
             if (one_read) {
-                launch_final(d, low1, ess, s, true);
+                if (!(first_whole && ge)) launch_final(d, low1, ess, s, true);
                 if (events) OPH_CHECK(hipEventRecord(c.ev1, s));
             }
             if (d.trk_ring)
