@@ -235,7 +235,47 @@ def cubical_leg(steps: int = 8, warmup: int = 2, cpu: bool = True):
         out["cpu_equals_gpu"] = None  # filled by main from the GPU result
         out["_cpu_low"] = cpu_low
         out["_b"] = b
+        out["reference_sample"] = cubical_reference_sample()
     return out
+
+
+def cubical_reference_sample(side: int = 40, budget_s: float = 6.0):
+    """The reference's own serial twist (reduce_col_host compiled from its
+    sources, fixed slots) on a bounded sample of the cubical workload: the
+    same field recipe on a side^3 grid, whose slot layout still fits 4 GiB
+    (64^3 would need about 28 GiB), beside the restatement and pms on the
+    same matrix. On cubical grids the reference's fixed slots (p = width x
+    longest input column, with the width the reduction's longest column
+    needs) dominate its time, so the restatement that cubical64's
+    cpu_baseline times is the stronger baseline."""
+    import oracle
+    from openph_amd import complexes as C
+
+    try:
+        oracle.build()
+        if not oracle.ref_available():
+            return None
+    except OSError:
+        return None
+    b = C.cubical_lower_star(C.smoothed_field(np.random.default_rng(0), side))
+    probe = oracle.twist_dyn(b)
+    mx = max(1, int(np.diff(b.colptr).max()))
+    cw = -(-max(probe.max_len, mx) // mx) + 1
+    slot_bytes = 2 * 4 * (b.m + 1) * cw * mx
+    if slot_bytes > (4 << 30):
+        return None
+    print(f"bench: cubical {side}^3 reference sample", file=sys.stderr, flush=True)
+    t_ref, res = _timed_runs(lambda: oracle.ref_run("twist", b, cw), budget_s)
+    t_port, _ = _timed_runs(lambda: oracle.twist_dyn(b), budget_s / 3)
+    wall, stats, _, _ = device_leg(b, f"cubical{side}", 10, 2, timed_steps=1)
+    ref_ms, port_ms = statistics.median(t_ref) * 1e3, statistics.median(t_port) * 1e3
+    gpu_ms = wall / 10 * 1e3
+    return {"workload": f"cubical {side}^3 lower star, same field recipe (m={b.m})",
+            "kind": "reference", "cores": 1,
+            "reference_ms": ref_ms, "reference_runs": len(t_ref), "slots_p": cw * mx,
+            "slot_bytes": slot_bytes, "restatement_ms": port_ms, "gpu_ms": gpu_ms,
+            "gpu_vs_reference": ref_ms / gpu_ms, "gpu_vs_restatement": port_ms / gpu_ms,
+            "reference_low_equals_restatement": bool(np.array_equal(res[0], probe.low))}
 
 
 def vr40_leg(steps: int = 50, warmup: int = 5, cpu: bool = True):
