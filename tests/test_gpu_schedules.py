@@ -349,3 +349,47 @@ def test_autotune_batch_checks_every_candidate_and_returns_a_working_setting():
         ref = oracle.pms(b)
         assert np.array_equal(got[o:o + b.m] - 1, ref.low)
         o += b.m
+
+
+def test_repeated_size_replays_the_whole_reduction_on_new_contents():
+    """A reduction of a size seen before replays one cached graph from the
+    set-up to the outputs, keyed on the buffers, not their contents: matrices
+    of the same size (VR 2-skeleta of 40 points are all m = 10,700, nnz =
+    31,200) written into the same device buffers must each reduce to their
+    own oracle result, a malformed one of that size must be refused, and the
+    next valid one must again be exact."""
+    import torch
+    dev = torch.device("cuda:0")
+    say("whole-reduction replays")
+    mats = [C.vr40(seed) for seed in (0, 1, 2)]
+    refs = [oracle.pms(b, snapshot=True) for b in mats]
+    m, nnz = mats[0].m, mats[0].nnz
+    assert all(b.m == m and b.nnz == nnz for b in mats)
+    R = torch.empty(nnz, dtype=torch.int32, device=dev)
+    Cc = torch.empty(nnz, dtype=torch.int32, device=dev)
+    low = torch.empty(m, dtype=torch.int32, device=dev)
+    ess = torch.empty(m, dtype=torch.int32, device=dev)
+    bad_r, bad_c = mats[1].coo1()
+    bad_r = bad_r.copy()
+    bad_r[100], bad_r[101] = bad_r[101], bad_r[100]  # rows out of order in a column
+    order = [0, 1, 0, 2, "bad", 1, 2, 0]
+    for k, which in enumerate(order):
+        if which == "bad":
+            R.copy_(torch.from_numpy(bad_r))
+            Cc.copy_(torch.from_numpy(bad_c))
+            with pytest.raises(openph_amd.OpenPHError) as e:
+                openph_amd.pms_device(R, Cc, m, low, ess)
+            assert e.value.status == -2, e.value.status  # OPENPH_EUNSORTED
+            continue
+        r, c = mats[which].coo1()
+        R.copy_(torch.from_numpy(r))
+        Cc.copy_(torch.from_numpy(c))
+        low.fill_(-7)
+        st = openph_amd.pms_device(R, Cc, m, low, ess)
+        torch.cuda.synchronize()
+        ref = refs[which]
+        assert np.array_equal(low.cpu().numpy() - 1, ref.low), (k, which)
+        assert np.array_equal(ess.cpu().numpy(), ref.ess), (k, which)
+        assert st.additions == ref.additions and st.merge_elems == ref.merge_elems, (k, which)
+        if k >= 2:  # the size repeats: the automatic schedule replays graphs
+            assert st.schedule == O.GRAPH
