@@ -22,7 +22,9 @@ Also reported, on the same JSON line:
 * cubical64: the same figures (rate, Phase II roofline, CPU twist) on the
   cubical lower-star filtration of a 64^3 grid, the largest single matrix.
 * batched / served: the 512-filtration batch and the request streams against
-  the plain drop-in; the batched leg carries its own Phase II roofline.
+  the plain drop-in; the batched leg carries its own Phase II roofline, each
+  served COO stream a whole-job one (its requests' B_alg over the burst's
+  span).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs a,b,...] [--no-cpu]
 """
@@ -644,7 +646,11 @@ def _served_stream(pool, seq, workers, seed, what):
     import openph_amd
 
     coo = [b.coo1() for b in pool]
-    ref = [openph_amd.reduce(b, "pms")[:2] for b in pool]  # warm-up and outputs alone
+    alone = [openph_amd.reduce(b, "pms") for b in pool]  # warm-up and outputs alone
+    ref = [a[:2] for a in alone]
+    # algorithmic bytes of each request, from its counters reduced alone
+    # (a batch member performs the same additions)
+    balg = [b_alg(a[2], b.m, b.nnz) for a, b in zip(alone, pool)]
     cols = sum(pool[i].m for i in seq)
     t0 = time.perf_counter()
     dev_ms = 0.0
@@ -658,8 +664,16 @@ def _served_stream(pool, seq, workers, seed, what):
                          borrow=False)
     qps = len(seq) / t_plain
     paced = _served_run(pool, coo, ref, seq, cols, qps, workers, np.random.default_rng(seed + 1))
+    job_bytes = float(sum(balg[i] for i in seq))
+    achieved = job_bytes / burst["span_s"] / 1e9
     return {
         "workload": what, "requests": len(seq), "columns": cols,
+        # whole job: the requests' B_alg (merges, COO in, outputs, 36 m per
+        # iteration) over the burst's span, every kernel of every batch
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "whole served job (all kernels of all batches)",
+                     "bytes": job_bytes, "span_s": burst["span_s"]},
         "plain_sequential_cols_per_s": cols / t_plain, "plain_sequential_s": t_plain,
         # the device time of those reductions (set-up to outputs, events on
         # the drop-in's stream): the rest of plain_sequential_s is host work
