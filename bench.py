@@ -35,7 +35,21 @@ import statistics
 import sys
 import time
 
-import numpy as np
+# Hardware queues for this process's streams, read once when HIP starts (so
+# set before anything initialises it; an environment asking for fewer than
+# 16, such as HIP's default of 4, is raised). The served legs run 4 worker
+# streams beside the streams the earlier legs created, and streams beyond
+# the queues share them, their kernels serialised (served vr_cubical after
+# the other legs: 2.9x the plain drop-in with 4 queues, 3.0x with 8, 4.6x
+# with 16).
+try:
+    _hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+except ValueError:
+    _hwq = 4
+if _hwq < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)

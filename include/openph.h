@@ -261,7 +261,13 @@ typedef struct openph_server openph_server;
 
 typedef struct openph_server_options {
     int device;              /* HIP device ordinal                           */
-    int workers;             /* concurrent batches (0 = 4, at most 16)       */
+    int workers;             /* concurrent batches (0 = 4, at most 16), each
+                                on a stream of its own. HIP gives a process
+                                4 hardware queues unless GPU_MAX_HW_QUEUES
+                                (read when HIP starts) asks for more; streams
+                                beyond them share queues and their kernels
+                                serialise, so a serving process raises it
+                                (16 for 4 workers beside other streams)      */
     int64_t worker_bytes;    /* device bytes a batch may claim (0 = 8 GiB)   */
     int max_batch;           /* requests per batch (0 = 1024)                */
     int max_queue;           /* queued requests before OPENPH_EBUSY
