@@ -222,7 +222,7 @@ def cubical_leg(steps: int = 8, warmup: int = 2, cpu: bool = True):
     dev_ms = statistics.median([s.device_ms for s in stats])
     value = b.m * steps / wall
     roof = phase2_roofline(tstats, b.m, f"Phase II: k_phase2_group<{st.lanes}, {st.group_lds}> "
-                                       "+ k_phase2_long")
+                                       "+ k_phase2_wide + k_phase2_long")
     # HBM bytes per Phase II pass from the committed counter runs
     tfile = os.path.join(ROOT, "profiles", "pms_cubical64_traffic.json")
     if os.path.exists(tfile):
