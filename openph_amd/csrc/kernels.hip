@@ -134,6 +134,50 @@ __device__ __forceinline__ void block_append(int *list, int *count, int value, b
     if (take) list[bbase + wbase[wid] + __popcll(b & ((1ull << lane) - 1))] = value;
 }
 
+// The per-column passes of an iteration take kItems consecutive columns per
+// thread. Their loads are issued together and waited for once: a pass over
+// columns costs the latency of its chain of dependent loads (column state,
+// then what it indexes), so no load waits behind another one it does not
+// depend on.
+//
+// v[k] = a[base + k] for base + k < n, else fill. A full run is one 16-byte
+// load: base is a multiple of kItems and every array starts 256-byte aligned.
+__device__ __forceinline__ void load_run(const int *a, int base, int n, int fill,
+                                         int (&v)[kItems]) {
+    static_assert(kItems == 4, "one int4 per run");
+    if (base + kItems <= n) {
+        const int4 x = *reinterpret_cast<const int4 *>(a + base);
+        v[0] = x.x;
+        v[1] = x.y;
+        v[2] = x.z;
+        v[3] = x.w;
+    } else {  // the last run: clamped loads, no branch per column
+        int raw[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; k++) raw[k] = a[base + k < n ? base + k : n - 1];
+#pragma unroll
+        for (int k = 0; k < kItems; k++) v[k] = base + k < n ? raw[k] : fill;
+    }
+}
+
+// v[k] = a[i[k]] for i[k] >= 0, else fill; the loads of a negative index
+// read a[0] (every state array has m + 1 >= 1 entries) and are dropped.
+__device__ __forceinline__ void gather_run(const int *a, const int (&i)[kItems], int fill,
+                                           int (&v)[kItems]) {
+    int raw[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; k++) raw[k] = a[i[k] > 0 ? i[k] : 0];
+#pragma unroll
+    for (int k = 0; k < kItems; k++) v[k] = i[k] >= 0 ? raw[k] : fill;
+}
+
+// Keeps the loads of v ahead of a branch on x that follows (the halt test),
+// so that they are in flight together with the load of x.
+__device__ __forceinline__ int issued(const int (&v)[kItems], int x) {
+    asm volatile("" : "+v"(x) : "v"(v[0]), "v"(v[1]), "v"(v[2]), "v"(v[3]));
+    return __builtin_amdgcn_readfirstlane(x);
+}
+
 __device__ __forceinline__ unsigned slot_size(int len, int slot0) {
     return (unsigned)(len > slot0 ? len : slot0);
 }
@@ -551,29 +595,39 @@ __global__ void k_phase0(Dev d, int a) {
 __global__ void __launch_bounds__(kBlock) k_apply1(Dev d) {
     __shared__ int lds[kBlock / 64];
     __shared__ int sbase;
-    if (d.ctrl->halt) return;  // uniform over the grid: no thread reaches the barrier
     const int base = blockIdx.x * kTile + threadIdx.x * kItems;
+    int cl[kItems], lo[kItems], cr[kItems];
+    const int halt = d.ctrl->halt;
+    load_run(d.classes, base, d.m, 1, cl);
+    load_run(d.low, base, d.m, -1, lo);
+    load_run(d.clear, base, d.m, 0, cr);
+    if (issued(lo, halt)) return;  // uniform over the grid: no thread reaches the barrier
+    int lt[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; k++) lt[k] = cr[k] || cl[k] != 0 ? -1 : lo[k];
+    int p[kItems];
+    gather_run(d.arglow, lt, -1, p);
+    if (base + kItems <= d.m) {
+        *reinterpret_cast<int4 *>(d.first + base) = make_int4(kIntMax, kIntMax, kIntMax, kIntMax);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kItems; k++)
+            if (base + k < d.m) d.first[base + k] = kIntMax;
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        const int t = base + k;
+        if (t < d.m && cr[k]) {
+            d.low[t] = -1;
+            d.classes[t] = 1;
+            d.clear[t] = 0;
+        }
+    }
     bool take[kItems];
     int cnt = 0;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
-        const int t = base + k;
-        take[k] = false;
-        if (t < d.m) {
-            int c = d.classes[t], lt = d.low[t];
-            if (d.clear[t]) {
-                lt = -1;
-                c = 1;
-                d.low[t] = -1;
-                d.classes[t] = 1;
-                d.clear[t] = 0;
-            }
-            d.first[t] = kIntMax;
-            if (c == 0 && lt > -1) {
-                const int p = d.arglow[lt];
-                take[k] = p > -1 && p < t;
-            }
-        }
+        take[k] = p[k] > -1 && p[k] < base + k;
         cnt += take[k] ? 1 : 0;
     }
     int tot;
@@ -721,7 +775,18 @@ constexpr int kHash = 2 * kTile;  // open addressing, load factor <= 1/2
 __global__ void __launch_bounds__(kBlock) k_first(Dev d) {
     __shared__ int hkey[kHash];
     __shared__ int hval[kHash];
-    if (d.ctrl->halt) return;
+    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
+    int lo[kItems], cl[kItems];
+    const int halt = d.ctrl->halt;
+    load_run(d.low, base, d.m, -1, lo);
+    load_run(d.classes, base, d.m, 1, cl);
+    if (issued(lo, halt)) return;
+    // only unknown columns whose low has no pivot yet take part
+#pragma unroll
+    for (int k = 0; k < kItems; k++)
+        if (cl[k] != 0) lo[k] = -1;
+    int p[kItems];
+    gather_run(d.arglow, lo, 0, p);
     // the iteration starts here: none of this kernel's blocks reads the
     // flags reset below
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -735,20 +800,17 @@ __global__ void __launch_bounds__(kBlock) k_first(Dev d) {
         hval[i] = kIntMax;
     }
     __syncthreads();
-    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
-        const int j = base + k;
-        if (j >= d.m) continue;
-        const int lj = d.low[j];
-        if (lj < 0 || d.classes[j] != 0 || d.arglow[lj] > -1) continue;
+        const int lj = lo[k];
+        if (lj < 0 || p[k] > -1) continue;
         unsigned h = ((unsigned)lj * 2654435761u) >> 21;  // 11 bits = kHash
         while (true) {
             const int old = atomicCAS(&hkey[h], -1, lj);
             if (old == -1 || old == lj) break;
             h = (h + 1) & (kHash - 1);
         }
-        atomicMin(&hval[h], j);
+        atomicMin(&hval[h], base + k);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kHash; i += kBlock) {
@@ -759,18 +821,17 @@ __global__ void __launch_bounds__(kBlock) k_first(Dev d) {
 static_assert(kHash == 2048, "k_first hashes to 11 bits");
 
 // dup = L when column j repeats an earlier column's low L, else -1;
-// fst = 1 when j is the first column with its low.
-__device__ __forceinline__ void dup_state(const Dev &d, int j, int lj, int &dup, int &fst) {
+// fst = 1 when j is the first column with its low. p = arglow[L] and
+// f = first[L], loaded together (f matters only when L has no pivot).
+__device__ __forceinline__ void dup_state(int j, int lj, int p, int f, int &dup, int &fst) {
     dup = -1;
     fst = 0;
     if (lj < 0) return;
-    const int p = d.arglow[lj];
     if (p > -1) {
         if (p == j) fst = 1;
         else dup = lj;
         return;
     }
-    const int f = d.first[lj];
     if (f == j) fst = 1;
     else if (f < j) dup = lj;
 }
@@ -791,26 +852,31 @@ __device__ __forceinline__ int segment_of(const int *off, int count, int x) {
 __global__ void __launch_bounds__(kBlock) k_tile(Dev d) {
     __shared__ int smax[kMaxSlots];
     __shared__ int sdup;
-    if (d.ctrl->halt) return;
+    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
+    int lo[kItems], sl[kItems];
+    const int halt = d.ctrl->halt;
+    load_run(d.low, base, d.m, -1, lo);
+    load_run(d.slot, base, d.m, 0, sl);
+    if (issued(lo, halt)) return;
+    int p[kItems], f[kItems];
+    gather_run(d.arglow, lo, -1, p);
+    gather_run(d.first, lo, kIntMax, f);
     if (threadIdx.x < kMaxSlots) smax[threadIdx.x] = -1;
     if (threadIdx.x == 0) sdup = 0;
     __syncthreads();
-    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
     int dupseen = 0;
     int mk = -1;  // member of the last repeating column (running batches)
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
-        int j = base + k;
-        if (j < d.m) {
-            int dup, fst;
-            dup_state(d, j, d.low[j], dup, fst);
-            if (dup > -1) {
-                atomicMax(&smax[d.slot[j]], dup);
-                dupseen = 1;
-                if (d.nmem) {
-                    if (mk < 0 || j >= d.mstart[mk + 1]) mk = segment_of(d.mstart, d.nmem, j);
-                    d.mdup[mk] = 1;
-                }
+        const int j = base + k;
+        int dup, fst;
+        dup_state(j, lo[k], p[k], f[k], dup, fst);
+        if (dup > -1) {
+            atomicMax(&smax[sl[k]], dup);
+            dupseen = 1;
+            if (d.nmem) {
+                if (mk < 0 || j >= d.mstart[mk + 1]) mk = segment_of(d.mstart, d.nmem, j);
+                d.mdup[mk] = 1;
             }
         }
     }
@@ -875,48 +941,47 @@ __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
     __shared__ int lds[kBlock / 64];
     __shared__ int exl[kMaxSlots][kBlock];
     __shared__ int s_tp[kMaxSlots];
-    const int nslots = d.ctrl->nslots;
+    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
+    int lo[kItems], sl[kItems], cl[kItems];
+    const Ctrl *c = d.ctrl;
+    const int halt0 = c->halt, err = c->err, iter = c->iter, any_dup = c->any_dup;
+    const int max_iters = c->max_iters, nslots = c->nslots;
+    load_run(d.low, base, d.m, -1, lo);
+    load_run(d.slot, base, d.m, 0, sl);
+    load_run(d.classes, base, d.m, 1, cl);
+    int h = issued(lo, halt0);
     if (FUSED) {
-        const Ctrl *c = d.ctrl;
-        int h = c->halt;
         if (!h) {
-            if (c->err) h = kHaltError;
-            else if (c->iter > 1 && !c->any_dup) h = kHaltConverged;
-            else if (c->iter > c->max_iters) h = kHaltIters;
+            if (err) h = kHaltError;
+            else if (iter > 1 && !any_dup) h = kHaltConverged;
+            else if (iter > max_iters) h = kHaltIters;
         }
         if (h) {
             if (blockIdx.x == 0 && threadIdx.x == 0) d.ctrl->halt = h;
             return;
         }
-        for (int sl = 0; sl < nslots && sl < kMaxSlots; sl++) {
+    } else if (h) {
+        return;
+    }
+    int p[kItems], f[kItems];
+    gather_run(d.arglow, lo, -1, p);
+    gather_run(d.first, lo, kIntMax, f);
+    if (FUSED) {
+        for (int s = 0; s < nslots && s < kMaxSlots; s++) {
             int mx = -1;
             for (int t = threadIdx.x; t < (int)blockIdx.x; t += kBlock) {
-                const int v = d.tile_max[t * kMaxSlots + sl];
+                const int v = d.tile_max[t * kMaxSlots + s];
                 mx = v > mx ? v : mx;
             }
             int tot;
             (void)block_exclusive_scan(mx, MaxOp<int>(), -1, lds, &tot);
-            if (threadIdx.x == 0) s_tp[sl] = tot;
+            if (threadIdx.x == 0) s_tp[s] = tot;
         }
         __syncthreads();
-    } else if (d.ctrl->halt) {
-        return;
     }
-    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
-    int lo[kItems], sl[kItems], dup[kItems], fst[kItems];
+    int dup[kItems], fst[kItems];
 #pragma unroll
-    for (int k = 0; k < kItems; k++) {
-        int j = base + k;
-        lo[k] = -1;
-        sl[k] = 0;
-        dup[k] = -1;
-        fst[k] = 0;
-        if (j < d.m) {
-            lo[k] = d.low[j];
-            sl[k] = d.slot[j];
-            dup_state(d, j, lo[k], dup[k], fst[k]);
-        }
-    }
+    for (int k = 0; k < kItems; k++) dup_state(base + k, lo[k], p[k], f[k], dup[k], fst[k]);
 #pragma unroll
     for (int s = 0; s < kMaxSlots; s++) {
         if (s < nslots) {
@@ -934,14 +999,14 @@ __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
     int marked = 0;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
-        int j = base + k;
-        if (j >= d.m || lo[k] < 0 || !fst[k]) continue;
+        const int j = base + k;
+        if (lo[k] < 0 || !fst[k]) continue;
         int c = exl[sl[k]][threadIdx.x];
 #pragma unroll
         for (int q = 0; q < k; q++)
             if (sl[q] == sl[k] && dup[q] > c) c = dup[q];
-        if (d.classes[j] == 0 && lo[k] > c) {
-            int L = lo[k];
+        if (cl[k] == 0 && lo[k] > c) {
+            const int L = lo[k];
             marked++;
             publish_pivot(d, L, j);
             d.classes[j] = -1;
@@ -2303,19 +2368,21 @@ __global__ void __launch_bounds__(kBlock) k_apply2(Dev d) {
     if (blockIdx.x == 0 && threadIdx.x == 0) iter_end(d.ctrl);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < d.m) {
-        int np = d.newpiv[t];
+        // column t's own entries, written by no other thread of the pass:
+        // loaded together before any store
+        const int np = d.newpiv[t], cr = d.clear[t], lo = d.low[t], al = d.arglow[t];
         if (np > -1) {
             publish_pivot(d, t, np);
             d.newpiv[t] = -1;
         }
-        if (d.clear[t]) {
+        if (cr) {
             d.low[t] = -1;
             d.classes[t] = 1;
             d.clear[t] = 0;
         }
-        int lt = d.low[t];
+        const int lt = cr ? -1 : lo;
         if (lt > -1) d.ess[lt] = 0;
-        int a = d.arglow[t];
+        const int a = np > -1 ? np : al;
         if (a > -1) {
             d.ess[a] = 0;
             d.ess[t] = 0;
