@@ -119,16 +119,19 @@ struct Dev {
 };
 
 // ---- launchers (kernels.hip) ----
+// Control block reset; for d.m > 0 also the length-presence map (slotmap).
 void launch_ctrl_init(const Dev &d, unsigned long long cap, int max_iters, hipStream_t s);
 // After the setup: pool capacity = max(requested, exact initial layout),
 // at most `alloc`.
 void launch_pool_cap(const Dev &d, unsigned long long requested, unsigned long long alloc,
                      hipStream_t s);
-// COO -> per-column state and pool layout (slot-granular segments).
+// COO -> per-column state, validation and pool layout (column j's segment:
+// len + slot0 entries at its COO start + j * slot0), lengths present.
 void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t s);
-// Malformed input found by the set-up: all columns made empty, the loop halted.
-void launch_setup_guard(const Dev &d, hipStream_t s);
-void launch_slots(const Dev &d, hipStream_t s);
+// After launch_setup: malformed input makes every column empty and halts
+// the loop; else left/beta and the slot numbers of the lengths present.
+void launch_settle(const Dev &d, hipStream_t s);
+// Slots of the columns, Phase 0 and its clears.
 void launch_phase0(const Dev &d, hipStream_t s);
 // iter + 1 and the per-iteration flags reset, unless halted. The pms loop
 // does this in k_first; the serial algorithms use it to clear the overflow

@@ -274,7 +274,11 @@ static void launch_scan(const Dev &d, int mode, unsigned *out, hipStream_t s, in
 // Setup: COO (1-based) -> pool + per-column state
 // ---------------------------------------------------------------------------
 
-__global__ void k_ctrl_init(Ctrl *c, unsigned long long cap, int max_iters) {
+// Also clears `zero` (n entries): the presence map of column lengths that
+// k_init_cols fills.
+__global__ void k_ctrl_init(Ctrl *c, unsigned long long cap, int max_iters, int *zero, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) zero[i] = 0;
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         c->pool_top = 0;
         c->pool_cap = cap;
@@ -329,7 +333,9 @@ __device__ __forceinline__ void fold_stripes(Ctrl *c) {
 }
 
 void launch_ctrl_init(const Dev &d, unsigned long long cap, int max_iters, hipStream_t s) {
-    k_ctrl_init<<<1, 64, 0, s>>>(d.ctrl, cap, max_iters);
+    const int n = d.m > 0 ? d.m + 2 : 0;  // slotmap has m + 2 entries
+    k_ctrl_init<<<n > 0 ? (n + kBlock - 1) / kBlock : 1, kBlock, 0, s>>>(d.ctrl, cap, max_iters,
+                                                                      d.slotmap, n);
 }
 
 // A caller-given pool size counts from the exact initial layout, which the
@@ -366,8 +372,11 @@ __device__ __forceinline__ int lower_bound_i(const int *a, int n, int key) {
 // column reaches Ctrl::maxlen with one atomic per block: one per lane on that
 // single address took 23 us of a Torus-100 setup.
 // Columns [a, d.m) and their COO entries [e0, e1) (global 1-based indices).
+// whole (a = 0, e0 = 0, nothing in the pool): column j's segment holds
+// len + slot0 entries, so its offset is its COO start + j * slot0 and the
+// layout needs no scan; its length is marked present in slotmap.
 __global__ void k_init_cols(Dev d, const int *__restrict__ rows1, const int *__restrict__ cols1,
-                            int a, int e0, int e1) {
+                            int a, int e0, int e1, int whole) {
     const int lane = lane_id();
     const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long jl = a + w * 63 + lane;
@@ -393,6 +402,13 @@ __global__ void k_init_cols(Dev d, const int *__restrict__ rows1, const int *__r
     const int j = (int)jl;
     d.off2[j] = (unsigned)lo;
     d.len[j] = n;
+    if (whole) {
+        d.off[j] = (unsigned)((unsigned long long)lo + (unsigned long long)j * d.slot0);
+        d.cap[j] = n + d.slot0;
+        if (n <= d.m) d.slotmap[n] = 1;  // (longer only in malformed input)
+        if (j == 0)
+            d.ctrl->pool_top = (unsigned long long)e1 + (unsigned long long)d.m * d.slot0;
+    }
     int l = n ? rows1[hi - 1] - 1 : -1;
     if (l < -1 || l >= d.m) l = -1;  // out of range; flagged by k_entries
     d.low[j] = l;
@@ -472,58 +488,70 @@ __global__ void k_beta(Dev d, int a) {
 }
 
 void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t s) {
-    launch_setup_range(d, 0, 0, d.nnz, rows1, cols1, s);
+    if (d.m > 0)
+        k_init_cols<<<init_cols_blocks(d.m), kBlock, 0, s>>>(d, rows1, cols1, 0, 0, d.nnz, 1);
+    if (d.nnz > 0)
+        k_entries<<<(d.nnz + kTile - 1) / kTile, kEntriesBlock, 0, s>>>(d, rows1, cols1, 0, 0,
+                                                                        d.nnz);
 }
 
-// After the set-up, for a host that does not wait for it: when the set-up
-// found the input malformed, every column is made empty (length 0, no low)
-// and the loop halted, so that the kernels queued behind it, which index by
-// column lengths and lows, touch nothing out of range. (Lows are in range
-// already: k_init_cols clamps them.) The host reports the error at its next
-// read of the control block.
-__global__ void k_setup_guard(Dev d) {
-    if (!d.ctrl->err) return;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j == 0) {
-        d.ctrl->maxlen = 0;
-        d.ctrl->halt = kHaltError;
+// After the set-up of the whole matrix, one pass:
+// * when the set-up found the input malformed, every column is made empty
+//   (length 0, no low) and the loop halted, so that the kernels queued
+//   behind it, which index by column lengths and lows, touch nothing out of
+//   range (lows are in range already: k_init_cols clamps them); the host
+//   reports the error at its next read of the control block;
+// * else beta (beta.cu:2-15) from left, the left rule into pivdesc, and in
+//   block 0 the present lengths numbered in increasing order: slot of a
+//   pseudo-dimension (dimensions.cu:61-63, pseudo-dim = len - 1).
+__global__ void __launch_bounds__(kBlock) k_settle(Dev d) {
+    __shared__ int lds[kBlock / 64];
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (d.ctrl->err) {
+        if (j == 0) {
+            d.ctrl->maxlen = 0;
+            d.ctrl->halt = kHaltError;
+        }
+        if (j < d.m) {
+            d.len[j] = 0;
+            d.low[j] = -1;
+        }
+        return;
+    }
+    if (blockIdx.x == 0) {
+        const int mx = d.ctrl->maxlen;
+        int run = 0;
+        for (int i0 = 0; i0 <= mx; i0 += kBlock) {
+            const int i = i0 + threadIdx.x;
+            const int present = i <= mx && d.slotmap[i] ? 1 : 0;
+            int tot;
+            const int ex = block_exclusive_scan(present, SumOp<int>(), 0, lds, &tot);
+            if (i <= mx) d.slotmap[i] = present ? run + ex : -1;
+            run += tot;
+        }
+        if (threadIdx.x == 0) {
+            d.ctrl->nslots = run;
+            if (run > kMaxSlots) atomicOr(&d.ctrl->err, kErrSlots);
+        }
     }
     if (j < d.m) {
-        d.len[j] = 0;
-        d.low[j] = -1;
+        const int l = d.left[j];
+        d.pivdesc[j].w = l;
+        if (l != kIntMax) atomicMax(&d.beta[l], j);
     }
 }
 
-void launch_setup_guard(const Dev &d, hipStream_t s) {
-    k_setup_guard<<<d.m > 0 ? (d.m + kBlock - 1) / kBlock : 1, kBlock, 0, s>>>(d);
+void launch_settle(const Dev &d, hipStream_t s) {
+    k_settle<<<d.m > 0 ? (d.m + kBlock - 1) / kBlock : 1, kBlock, 0, s>>>(d);
 }
 
 void launch_setup_range(const Dev &d, int a, int e0, int e1, const int *rows1, const int *cols1,
                         hipStream_t s) {
     const int ge = (e1 - e0 + kTile - 1) / kTile;
     if (d.m > a)
-        k_init_cols<<<init_cols_blocks(d.m - a), kBlock, 0, s>>>(d, rows1, cols1, a, e0, e1);
+        k_init_cols<<<init_cols_blocks(d.m - a), kBlock, 0, s>>>(d, rows1, cols1, a, e0, e1, 0);
     launch_scan(d, 0, d.off, s, a, a > 0);
     if (e1 > e0) k_entries<<<ge, kEntriesBlock, 0, s>>>(d, rows1, cols1, a, e0, e1);
-}
-
-// Pseudo-dimension slots: pseudo-dim = len - 1 (dimensions.cu:61-63). The
-// distinct values present are numbered in increasing order.
-__global__ void k_presence(Dev d) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= d.m) return;
-    d.slotmap[d.len[j]] = 1;
-}
-
-__global__ void k_slotmap(Dev d) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int n = 0, mx = d.ctrl->maxlen;
-    for (int i = 0; i <= mx; i++) {
-        if (d.slotmap[i]) d.slotmap[i] = n++;
-        else d.slotmap[i] = -1;
-    }
-    d.ctrl->nslots = n;
-    if (n > kMaxSlots) atomicOr(&d.ctrl->err, kErrSlots);
 }
 
 __global__ void k_assign_slot(Dev d, int a) {
@@ -533,20 +561,6 @@ __global__ void k_assign_slot(Dev d, int a) {
     d.slot[j] = s < 0 ? 0 : (s >= kMaxSlots ? kMaxSlots - 1 : s);
 }
 
-__global__ void k_zero_i(int *p, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-
-void launch_slots(const Dev &d, hipStream_t s) {
-    int gm = (d.m + kBlock - 1) / kBlock;
-    int g1 = (d.m + 1 + kBlock - 1) / kBlock;
-    k_zero_i<<<g1, kBlock, 0, s>>>(d.slotmap, d.m + 1);
-    k_beta<<<gm, kBlock, 0, s>>>(d, 0);
-    k_presence<<<gm, kBlock, 0, s>>>(d);
-    k_slotmap<<<1, 64, 0, s>>>(d);
-    k_assign_slot<<<gm, kBlock, 0, s>>>(d, 0);
-}
 
 // Columns [a, d.m) added to a running batch: left/beta and the slots from a
 // slot map the host keeps (Ctrl::nslots set with it).
@@ -572,9 +586,14 @@ __device__ __forceinline__ void publish_pivot(const Dev &d, int l, int j) {
 // Phase 0: mark_pivots_and_clear (pms_functions.cu:33-53)
 // ---------------------------------------------------------------------------
 
-__global__ void k_phase0(Dev d, int a) {
+// assign: also the column's slot, from the numbered lengths (k_settle)
+__global__ void k_phase0(Dev d, int a, int assign) {
     int j = a + blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= d.m) return;
+    if (assign) {
+        const int sl = d.slotmap[d.len[j]];
+        d.slot[j] = sl < 0 ? 0 : (sl >= kMaxSlots ? kMaxSlots - 1 : sl);
+    }
     int lj = d.low[j];
     if (lj > -1 && lj == d.beta[j]) {
         d.classes[j] = -1;
@@ -645,7 +664,7 @@ __global__ void __launch_bounds__(kBlock) k_apply1(Dev d) {
 void launch_phase0(const Dev &d, hipStream_t s) {
     int gm = (d.m + kBlock - 1) / kBlock;
     if (!d.m) return;
-    k_phase0<<<gm, kBlock, 0, s>>>(d, 0);
+    k_phase0<<<gm, kBlock, 0, s>>>(d, 0, 1);
     k_apply1<<<d.ntiles, kBlock, 0, s>>>(d);
 }
 
@@ -666,7 +685,7 @@ __global__ void k_clear_range(Dev d, int a) {
 void launch_phase0_range(const Dev &d, int a, hipStream_t s) {
     const int gm = (d.m - a + kBlock - 1) / kBlock;
     if (gm <= 0) return;
-    k_phase0<<<gm, kBlock, 0, s>>>(d, a);
+    k_phase0<<<gm, kBlock, 0, s>>>(d, a, 0);
     k_clear_range<<<gm, kBlock, 0, s>>>(d, a);
 }
 
@@ -933,15 +952,40 @@ __global__ void __launch_bounds__(1024) k_tilescan(Dev d) {
     }
 }
 
+// Up to this many tiles each k_mark block reads the earlier tiles' maxima
+// itself (at most 2 tiles per thread) instead of a k_tilescan pass.
+constexpr int kFuseTiles = 512;
+static_assert(kFuseTiles <= 2 * kBlock, "two earlier tiles per thread");
+static_assert(kMaxSlots % 4 == 0, "a tile's slot maxima as int4");
+
+// The 16 slot maxima of tile t (64 contiguous bytes), or -1 when !valid.
+__device__ __forceinline__ void load_tile_slots(const int *tm, int t, bool valid,
+                                                int (&v)[kMaxSlots]) {
+    const int4 *q = reinterpret_cast<const int4 *>(tm + (valid ? t : 0) * kMaxSlots);
+#pragma unroll
+    for (int i = 0; i < kMaxSlots / 4; i++) {
+        const int4 x = q[i];
+        v[4 * i] = valid ? x.x : -1;
+        v[4 * i + 1] = valid ? x.y : -1;
+        v[4 * i + 2] = valid ? x.z : -1;
+        v[4 * i + 3] = valid ? x.w : -1;
+    }
+}
+
 // FUSED (few tiles): k_tilescan's work done here, by every block from the
 // same inputs: the loop's exit (block 0 writes it) and the maximum over the
 // earlier tiles, read by each block itself. One launch less per iteration.
+// All slots are scanned together: one wave scan per slot in registers and a
+// single block barrier for the lot.
 template <bool FUSED>
 __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
-    __shared__ int lds[kBlock / 64];
+    constexpr int NW = kBlock / 64;
     __shared__ int exl[kMaxSlots][kBlock];
-    __shared__ int s_tp[kMaxSlots];
+    __shared__ int s_agg[kMaxSlots][NW];
+    __shared__ int s_tm[kMaxSlots][NW];
+    __shared__ int s_cnt[NW];
     const int base = blockIdx.x * kTile + threadIdx.x * kItems;
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
     int lo[kItems], sl[kItems], cl[kItems];
     const Ctrl *c = d.ctrl;
     const int halt0 = c->halt, err = c->err, iter = c->iter, any_dup = c->any_dup;
@@ -966,22 +1010,24 @@ __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
     int p[kItems], f[kItems];
     gather_run(d.arglow, lo, -1, p);
     gather_run(d.first, lo, kIntMax, f);
+    // per slot, the largest duplicate low of the earlier tiles: FUSED, the
+    // maximum over this thread's share of them (reduced over the block
+    // below); else k_tilescan's prefix for this tile
+    int tm[kMaxSlots];
     if (FUSED) {
-        for (int s = 0; s < nslots && s < kMaxSlots; s++) {
-            int mx = -1;
-            for (int t = threadIdx.x; t < (int)blockIdx.x; t += kBlock) {
-                const int v = d.tile_max[t * kMaxSlots + s];
-                mx = v > mx ? v : mx;
-            }
-            int tot;
-            (void)block_exclusive_scan(mx, MaxOp<int>(), -1, lds, &tot);
-            if (threadIdx.x == 0) s_tp[s] = tot;
-        }
-        __syncthreads();
+        const int t0 = threadIdx.x, t1 = threadIdx.x + kBlock;
+        int v1[kMaxSlots];
+        load_tile_slots(d.tile_max, t0, t0 < (int)blockIdx.x, tm);
+        load_tile_slots(d.tile_max, t1, t1 < (int)blockIdx.x, v1);
+#pragma unroll
+        for (int s = 0; s < kMaxSlots; s++) tm[s] = v1[s] > tm[s] ? v1[s] : tm[s];
+    } else {
+        load_tile_slots(d.tile_pre, blockIdx.x, true, tm);
     }
     int dup[kItems], fst[kItems];
 #pragma unroll
     for (int k = 0; k < kItems; k++) dup_state(base + k, lo[k], p[k], f[k], dup[k], fst[k]);
+    // exclusive max scan of the duplicate lows per slot over the block
 #pragma unroll
     for (int s = 0; s < kMaxSlots; s++) {
         if (s < nslots) {
@@ -989,13 +1035,33 @@ __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
 #pragma unroll
             for (int k = 0; k < kItems; k++)
                 if (sl[k] == s && dup[k] > agg) agg = dup[k];
-            int tot;
-            int ex = block_exclusive_scan(agg, MaxOp<int>(), -1, lds, &tot);
-            int tp = FUSED ? s_tp[s] : d.tile_pre[blockIdx.x * kMaxSlots + s];
-            exl[s][threadIdx.x] = ex > tp ? ex : tp;
+            const int incl = wave_inclusive_scan(agg, MaxOp<int>(), -1);
+            exl[s][threadIdx.x] = dpp_or<0x138, 0xF>(-1, incl);  // wave_shr:1
+            const int wtot = __builtin_amdgcn_readlane(incl, 63);
+            int wtm = -1;
+            if (FUSED)
+                wtm = __builtin_amdgcn_readlane(wave_inclusive_scan(tm[s], MaxOp<int>(), -1), 63);
+            if (lane == 0) {
+                s_agg[s][wid] = wtot;
+                s_tm[s][wid] = wtm;
+            }
         }
     }
     __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kMaxSlots; s++) {
+        if (s < nslots) {
+            int pre = exl[s][threadIdx.x];
+            int tp = FUSED ? -1 : tm[s];
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                const int x = s_agg[s][w];
+                if (w < wid && x > pre) pre = x;
+                if (FUSED && s_tm[s][w] > tp) tp = s_tm[s][w];
+            }
+            exl[s][threadIdx.x] = pre > tp ? pre : tp;
+        }
+    }
     int marked = 0;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
@@ -1016,19 +1082,21 @@ __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
         }
     }
     // one atomic per block on the shared counter
-    int tot;
-    (void)block_exclusive_scan(marked, SumOp<int>(), 0, lds, &tot);
-    if (threadIdx.x == 0 && tot) atomicAdd(&d.ctrl->marks, tot);
+    const int wm = __builtin_amdgcn_readlane(wave_inclusive_scan(marked, SumOp<int>(), 0), 63);
+    if (lane == 0) s_cnt[wid] = wm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) tot += s_cnt[w];
+        if (tot) atomicAdd(&d.ctrl->marks, tot);
+    }
 }
 
 void launch_phase1_detect(const Dev &d, hipStream_t s) {
     k_first<<<d.ntiles, kBlock, 0, s>>>(d);
     k_tile<<<d.ntiles, kBlock, 0, s>>>(d);
 }
-
-// Up to this many tiles each k_mark block reads the earlier tiles' maxima
-// itself (at most 2 loads per thread and slot) instead of a k_tilescan pass.
-constexpr int kFuseTiles = 512;
 
 void launch_phase1_mark(const Dev &d, hipStream_t s) {
     if (d.ntiles <= kFuseTiles) {

@@ -233,8 +233,7 @@ static void enqueue_setup(const Dev &d, const WholeArgs &w, hipStream_t s) {
     launch_ctrl_init(d, (unsigned long long)w.lcap, w.max_iters, s);
     launch_setup(d, w.rows1, w.cols1, s);
     if (w.pool_elems) launch_pool_cap(d, w.pool_elems, w.lcap, s);
-    launch_setup_guard(d, s);
-    launch_slots(d, s);
+    launch_settle(d, s);
     launch_phase0(d, s);
 }
 
@@ -468,10 +467,7 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
         if (cfg.pool_elems) launch_pool_cap(d, cfg.pool_elems, lcap, s);
     }
     int maxlen = 0;
-    if (ahead) {
-        // the error, and the pool's size, come with the first read
-        if (!whole) launch_setup_guard(d, s);
-    } else {
+    if (!ahead) {
         rc = read_ctrl(c, s);  // also completes the low_true upload from pageable memory
         if (rc) return rc;
         if ((rc = err_from_bits(c.h_ctrl->err))) return rc;
@@ -507,7 +503,9 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
         if (ring) launch_track_iter(d, s);  // the state before Phase 0 (iter 0)
         else if ((rc = track_state(c, tr, 0, s, m, ess_true))) return rc;
     }
-    if (!whole) launch_slots(d, s);
+    // with `ahead` the error, and the pool's size, come with the first
+    // read; k_settle makes malformed input inert meanwhile
+    if (!whole) launch_settle(d, s);
     // Phase II lanes per column: short input columns (simplicial complexes
     // of small dimension) take 8 lanes with 80-entry LDS buffers (a 16-entry
     // pivot buffer: 22 KB per block, so seven blocks share a CU's 160 KB),
