@@ -353,40 +353,93 @@ void launch_pool_cap(const Dev &d, unsigned long long requested, unsigned long l
     k_pool_cap<<<1, 64, 0, s>>>(d.ctrl, requested, alloc);
 }
 
-__device__ __forceinline__ int lower_bound_i(const int *a, int n, int key) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (a[mid] < key) lo = mid + 1;
-        else hi = mid;
+// The first positions of sorted cols1[lo, hi) whose column (1-based) is >=
+// ka and >= kb, found together by one wave: lanes 0-31 search for ka and
+// lanes 32-63 for kb, each half probing 32 evenly spaced positions per step
+// (a 32-ary search: 4 dependent loads for half a million entries where a
+// binary search takes 19). In unsorted input the results are some positions
+// in [lo, hi]. Every lane of the wave calls it.
+__device__ __forceinline__ void wave_lower_bounds(const int *__restrict__ cols1, int lo, int hi,
+                                                  int ka, int kb, int &ra, int &rb) {
+    const int lane = lane_id(), q = lane & 31;
+    const bool upper = lane >= 32;
+    const int key = upper ? kb : ka;
+    long long l = lo, r = hi;  // the answer lies in [l, r]
+    while (true) {
+        const bool more = r - l > 32;
+        if (!__ballot(more)) break;
+        const long long step = more ? (r - l + 31) / 32 : 0;
+        const long long pos = l + q * step;
+        const bool less = more && pos < r && cols1[pos] < key;
+        const unsigned long long bal = __ballot(less);
+        const int nl = __popcll(upper ? bal >> 32 : bal & 0xffffffffull);
+        if (more) {
+            const long long cand = l + nl * step;
+            l = nl ? l + (nl - 1) * step + 1 : l;
+            r = cand < r ? cand : r;
+        }
     }
-    return lo;
+    const long long pos = l + q;
+    const unsigned long long bal = __ballot(pos < r && cols1[pos] < key);
+    const int res = (int)l + __popcll(upper ? bal >> 32 : bal & 0xffffffffull);
+    ra = __shfl(res, 0, 64);
+    rb = __shfl(res, 32, 64);
 }
 
-// One thread per column: column extent by binary search in the sorted cols,
-// initial state (openph.cu:86-90 fills, kernels.cu:40-52 compute_low). The
-// COO start of the column goes to off2 until the pool layout is known.
-// A wave covers 63 columns: lane l searches the start of column 63w + l, and
-// lane 63 that of the next wave's first column, so every column's end comes
-// from its right neighbour lane and each lane searches once. The longest
-// column reaches Ctrl::maxlen with one atomic per block: one per lane on that
-// single address took 23 us of a Torus-100 setup.
+// Column extents and initial state (openph.cu:86-90 fills, kernels.cu:40-52
+// compute_low), one wave per 64 columns: the wave finds where its columns'
+// COO entries begin and end (wave_lower_bounds), then reads those entries
+// 64 at a time and each column boundary among them gives the starts of the
+// columns up to it. The COO start of a column goes to off2. The longest
+// column reaches Ctrl::maxlen with one atomic per block: one per lane on
+// that single address took 23 us of a Torus-100 setup.
 // Columns [a, d.m) and their COO entries [e0, e1) (global 1-based indices).
 // whole (a = 0, e0 = 0, nothing in the pool): column j's segment holds
 // len + slot0 entries, so its offset is its COO start + j * slot0 and the
 // layout needs no scan; its length is marked present in slotmap.
-__global__ void k_init_cols(Dev d, const int *__restrict__ rows1, const int *__restrict__ cols1,
-                            int a, int e0, int e1, int whole) {
-    const int lane = lane_id();
-    const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long long jl = a + w * 63 + lane;
-    const int key = jl < d.m ? (int)jl + 1 : d.m + 1;
+__global__ void __launch_bounds__(kBlock) k_init_cols(Dev d, const int *__restrict__ rows1,
+                                                      const int *__restrict__ cols1, int a,
+                                                      int e0, int e1, int whole) {
+    constexpr int NW = kBlock / 64;
+    constexpr int U = 4;  // windows of 64 entries read at once
+    __shared__ int s_start[NW][65];
     __shared__ int s_max;
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
+    const long long jw = a + (((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64;
+    const int key0 = jw < d.m ? (int)jw + 1 : d.m + 1;       // the wave's first column, 1-based
+    const int keyE = jw + 64 < d.m ? (int)jw + 65 : d.m + 1;  // one past its last
     if (threadIdx.x == 0) s_max = 0;
+    int lo0, loE;
+    wave_lower_bounds(cols1, e0, e1, key0, keyE, lo0, loE);
+    if (loE < lo0) loE = lo0;  // unsorted input; flagged by k_entries
+    s_start[wid][lane] = loE;
+    if (lane == 0) s_start[wid][64] = loE;
+    // column c1 > c0 at entry e: the columns (c0, c1] of the wave start at e
+    const int nwin = (loE - lo0 + 63) >> 6;
+    for (int w0 = 0; w0 < nwin; w0 += U) {
+        int c[U], pc[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int e = lo0 + ((w0 + u) << 6) + lane;
+            const int ec = e < loE ? e : loE - 1;
+            c[u] = cols1[ec];
+            pc[u] = cols1[ec > lo0 ? ec - 1 : lo0];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int e = lo0 + ((w0 + u) << 6) + lane;
+            if (e < loE) {
+                const int from = e > lo0 && pc[u] + 1 > key0 ? pc[u] + 1 : key0;
+                const int to = c[u] < keyE - 1 ? c[u] : keyE - 1;
+                for (int k = from; k <= to; k++) s_start[wid][k - key0] = e;
+            }
+        }
+    }
     __syncthreads();
-    const int lo = e0 + lower_bound_i(cols1 + e0, e1 - e0, key);
-    int hi = __shfl_down(lo, 1, 64);
-    const bool live = lane != 63 && jl < d.m;
+    const long long jl = jw + lane;
+    const bool live = jl < d.m;
+    const int lo = s_start[wid][lane];
+    int hi = s_start[wid][lane + 1];
     if (hi < lo) hi = lo;  // unsorted input; flagged by k_entries
     const int n = live ? hi - lo : 0;
     int wmax = n;
@@ -424,7 +477,7 @@ __global__ void k_init_cols(Dev d, const int *__restrict__ rows1, const int *__r
 }
 
 static int init_cols_blocks(int m) {
-    const long long waves = ((long long)m + 62) / 63;
+    const long long waves = ((long long)m + 63) / 64;
     return (int)((waves * 64 + kBlock - 1) / kBlock);
 }
 
