@@ -481,18 +481,31 @@ static int init_cols_blocks(int m) {
     return (int)((waves * 64 + kBlock - 1) / kBlock);
 }
 
-// Validation, pool copy (0-based) and left[] by atomicMin, kTile entries per
-// block. A row occurs in many columns (an edge in every triangle on it), so
-// the minimum column per row is first taken in an LDS hash table and then
-// issued once per distinct row and block.
-constexpr int kEntriesBlock = kTile;  // one entry per thread
+// Validation, pool copy (0-based) and left[] by atomicMin. A row occurs in
+// many columns (an edge of a Vietoris-Rips complex in every triangle on it,
+// a few hundred columns apart), and device-scope atomics run at some 20 G/s
+// for the whole GPU, slower still on one address. So a block takes
+// kEntriesTile consecutive entries, the minimum column per row first in an
+// LDS hash table, and issues one atomic per distinct row: 219 K atomics
+// instead of 302 K (1,024-entry tiles) on Torus-100, 12.9 M instead of
+// 22.8 M on a batch of 512 VR 2-skeleta. Larger tiles cut the atomics
+// further, but their fewer blocks take longer than the atomics they save
+// (measured at 8 and 16 entries per thread). A row that finds no slot within
+// kProbe probes (cubical grids: few repeats, many distinct rows) takes its
+// atomic at once.
+constexpr int kEntriesBlock = 1024;
+constexpr int kEntriesPer = 4;  // entries per thread
+constexpr int kEntriesTile = kEntriesBlock * kEntriesPer;
+constexpr int kEntriesHash = 8192;  // slots (key and value: 64 KB of LDS)
+constexpr int kProbe = 8;
 
 // Entries [e0, e1), of columns [a, d.m): a row below a (another matrix of
 // a batch) is out of range.
 __global__ void __launch_bounds__(kEntriesBlock) k_entries(Dev d, const int *__restrict__ rows1,
                                                           const int *__restrict__ cols1, int a,
                                                           int e0, int e1) {
-    constexpr int H = 2 * kTile;
+    constexpr int H = kEntriesHash;
+    constexpr int R = 4;  // entries whose loads are issued together
     __shared__ int hkey[H];
     __shared__ int hval[H];
     for (int i = threadIdx.x; i < H; i += kEntriesBlock) {
@@ -501,28 +514,53 @@ __global__ void __launch_bounds__(kEntriesBlock) k_entries(Dev d, const int *__r
     }
     __syncthreads();
     int bad = 0;
-    for (int k = 0; k < kTile / kEntriesBlock; k++) {
-        const int e = e0 + blockIdx.x * kTile + k * kEntriesBlock + threadIdx.x;  // coalesced
-        if (e >= e1) continue;
-        const int c = cols1[e] - 1, r = rows1[e] - 1;
-        int b = 0;
-        if (c < a || c >= d.m || r < a || r >= c) b |= kErrRange;
-        if (e > e0) {
-            const int pc = cols1[e - 1] - 1;
-            if (c < pc) b |= kErrOrder;
-            else if (c == pc && rows1[e - 1] - 1 >= r) b |= kErrRows;
+    const long long tile0 = (long long)e0 + (long long)blockIdx.x * kEntriesTile;
+    for (int k0 = 0; k0 < kEntriesPer; k0 += R) {
+        int c[R], r[R], b[R];
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+            const long long el = tile0 + (long long)(k0 + u) * kEntriesBlock + threadIdx.x;
+            b[u] = el < e1 ? 0 : -1;  // -1: no entry
+            const int e = el < e1 ? (int)el : e1 - 1;  // coalesced
+            c[u] = cols1[e] - 1;
+            r[u] = rows1[e] - 1;
+            const int pc = cols1[e > e0 ? e - 1 : e] - 1, pr = rows1[e > e0 ? e - 1 : e] - 1;
+            if (b[u] == 0) {
+                if (c[u] < a || c[u] >= d.m || r[u] < a || r[u] >= c[u]) b[u] |= kErrRange;
+                if (e > e0) {
+                    if (c[u] < pc) b[u] |= kErrOrder;
+                    else if (c[u] == pc && pr >= r[u]) b[u] |= kErrRows;
+                }
+                bad |= b[u];
+            }
         }
-        bad |= b;
-        if (b & kErrRange) continue;
-        const int q = e - (int)d.off2[c];
-        if (q >= 0 && q < d.len[c]) d.pool[d.off[c] + q] = r;
-        unsigned h = ((unsigned)r * 2654435761u) >> 21;
-        while (true) {
-            const int old = atomicCAS(&hkey[h], -1, r);
-            if (old == -1 || old == r) break;
-            h = (h + 1) & (H - 1);
+        int o2[R], ln[R], of[R];
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+            const int cc = b[u] & kErrRange ? a : c[u];  // a valid column for the loads
+            o2[u] = (int)d.off2[cc];
+            ln[u] = d.len[cc];
+            of[u] = (int)d.off[cc];
         }
-        atomicMin(&hval[h], c);
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+            if (b[u] & kErrRange) continue;  // (and no entry: b = -1)
+            const int e = (int)(tile0 + (long long)(k0 + u) * kEntriesBlock + threadIdx.x);
+            const int q = e - o2[u];
+            if (q >= 0 && q < ln[u]) d.pool[(unsigned)of[u] + q] = r[u];
+            unsigned h = ((unsigned)r[u] * 2654435761u) >> 19;  // 13 bits = H
+            bool in = false;
+            for (int p = 0; p < kProbe; p++) {
+                const int old = atomicCAS(&hkey[h], -1, r[u]);
+                if (old == -1 || old == r[u]) {
+                    atomicMin(&hval[h], c[u]);
+                    in = true;
+                    break;
+                }
+                h = (h + 1) & (H - 1);
+            }
+            if (!in) atomicMin(&d.left[r[u]], c[u]);
+        }
     }
     if (bad) atomicOr(&d.ctrl->err, bad);
     __syncthreads();
@@ -530,6 +568,11 @@ __global__ void __launch_bounds__(kEntriesBlock) k_entries(Dev d, const int *__r
         const int key = hkey[i];
         if (key > -1) atomicMin(&d.left[key], hval[i]);
     }
+}
+static_assert(kEntriesHash == 1 << 13, "k_entries hashes to 13 bits");
+
+static int entries_blocks(long long n) {
+    return (int)((n + kEntriesTile - 1) / kEntriesTile);
 }
 
 __global__ void k_beta(Dev d, int a) {
@@ -544,8 +587,7 @@ void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t 
     if (d.m > 0)
         k_init_cols<<<init_cols_blocks(d.m), kBlock, 0, s>>>(d, rows1, cols1, 0, 0, d.nnz, 1);
     if (d.nnz > 0)
-        k_entries<<<(d.nnz + kTile - 1) / kTile, kEntriesBlock, 0, s>>>(d, rows1, cols1, 0, 0,
-                                                                        d.nnz);
+        k_entries<<<entries_blocks(d.nnz), kEntriesBlock, 0, s>>>(d, rows1, cols1, 0, 0, d.nnz);
 }
 
 // After the set-up of the whole matrix, one pass:
@@ -556,7 +598,10 @@ void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t 
 //   reports the error at its next read of the control block;
 // * else beta (beta.cu:2-15) from left, the left rule into pivdesc, and in
 //   block 0 the present lengths numbered in increasing order: slot of a
-//   pseudo-dimension (dimensions.cu:61-63, pseudo-dim = len - 1).
+//   pseudo-dimension (dimensions.cu:61-63, pseudo-dim = len - 1). (Finding
+//   beta[j] among column j's own entries, which holds every i with
+//   left[i] = j, instead of an atomic per column took longer: the gathers of
+//   left[] cost more than the atomics, 10 against 3.4 us on Torus-100.)
 __global__ void __launch_bounds__(kBlock) k_settle(Dev d) {
     __shared__ int lds[kBlock / 64];
     const int j = blockIdx.x * kBlock + threadIdx.x;
@@ -600,7 +645,7 @@ void launch_settle(const Dev &d, hipStream_t s) {
 
 void launch_setup_range(const Dev &d, int a, int e0, int e1, const int *rows1, const int *cols1,
                         hipStream_t s) {
-    const int ge = (e1 - e0 + kTile - 1) / kTile;
+    const int ge = entries_blocks((long long)e1 - e0);
     if (d.m > a)
         k_init_cols<<<init_cols_blocks(d.m - a), kBlock, 0, s>>>(d, rows1, cols1, a, e0, e1, 0);
     launch_scan(d, 0, d.off, s, a, a > 0);
