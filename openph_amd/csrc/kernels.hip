@@ -757,13 +757,12 @@ __global__ void __launch_bounds__(kBlock) k_apply1(Dev d) {
         if (take[k]) d.work[o++] = base + k;
 }
 
-// Phase 0 and its clears; k_apply1 also resets first[] for Phase I (the
-// worklist it builds is dropped by the first k_iter_begin).
+// Phase 0; the first iteration's k_first applies its clears (first[] is
+// reset by k_init_cols).
 void launch_phase0(const Dev &d, hipStream_t s) {
     int gm = (d.m + kBlock - 1) / kBlock;
     if (!d.m) return;
     k_phase0<<<gm, kBlock, 0, s>>>(d, 0, 1);
-    k_apply1<<<d.ntiles, kBlock, 0, s>>>(d);
 }
 
 // Phase 0 of columns [a, d.m) added to a running batch, its clears and the
@@ -894,14 +893,28 @@ __global__ void __launch_bounds__(kBlock) k_first(Dev d) {
     __shared__ int hval[kHash];
     const int base = blockIdx.x * kTile + threadIdx.x * kItems;
     int lo[kItems], cl[kItems];
+    int cr[kItems];
     const int halt = d.ctrl->halt;
     load_run(d.low, base, d.m, -1, lo);
     load_run(d.classes, base, d.m, 1, cl);
+    load_run(d.clear, base, d.m, 0, cr);
     if (issued(lo, halt)) return;
+    // Phase 0's clears (clear_positives, pms_functions.cu:131-140): pending
+    // only in the first iteration, every later clear is applied in the
+    // iteration that sets it
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        if (cr[k]) {
+            const int j = base + k;
+            d.low[j] = -1;
+            d.classes[j] = 1;
+            d.clear[j] = 0;
+        }
+    }
     // only unknown columns whose low has no pivot yet take part
 #pragma unroll
     for (int k = 0; k < kItems; k++)
-        if (cl[k] != 0) lo[k] = -1;
+        if (cl[k] != 0 || cr[k]) lo[k] = -1;
     int p[kItems];
     gather_run(d.arglow, lo, 0, p);
     // the iteration starts here: none of this kernel's blocks reads the

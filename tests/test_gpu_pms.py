@@ -398,3 +398,50 @@ def test_batch_rejects_index_outside_its_matrix():
     rc = L.openph_pms_batch(2, ip(ms), ip(nnzs), ip(rows), ip(cols), None, ip(low), ip(ess), None)
     assert rc == -1
     del bad
+
+
+def _polygons_boundary(isolated, sizes, faces_per_cycle=3):
+    """A 2-dimensional cell complex in filtration order: `isolated` vertices
+    with no cofaces, then for each k in `sizes` a cycle of k vertices and k
+    edges, all vertices first, then all edges, then `faces_per_cycle` 2-cells
+    per cycle, each bounded by all k edges of its cycle (a boundary matrix:
+    every vertex lies on two edges of its cycle). Input columns run from 0 to
+    max(sizes) entries, after a run of `isolated` + sum(sizes) empty ones."""
+    nv = isolated + sum(sizes)
+    vbase, v = [], isolated
+    for k in sizes:
+        vbase.append(v)
+        v += k
+    cols = [[] for _ in range(nv)]
+    ebase, e = [], nv
+    for k, vb in zip(sizes, vbase):
+        ebase.append(e)
+        for i in range(k):
+            a, b = vb + i, vb + (i + 1) % k
+            cols.append(sorted((a, b)))
+            e += 1
+    for k, eb in zip(sizes, ebase):
+        for _ in range(faces_per_cycle):
+            cols.append(list(range(eb, eb + k)))
+    counts = np.array([len(c) for c in cols], dtype=np.int64)
+    colptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    rows = np.array([r for c in cols for r in c], dtype=np.int32)
+    dim = np.array([0] * nv + [1] * (e - nv) + [2] * (len(cols) - e), dtype=np.int32)
+    return C.Boundary(colptr=colptr, rows=rows, dim=dim,
+                      value=np.arange(len(cols), dtype=np.float64), name="polygons")
+
+
+def test_setup_long_empty_runs_and_long_input_columns():
+    """The set-up's column extents and pool layout where a wave's columns
+    have no entries at all (thousands of empty columns in a row) and where
+    one wave's columns hold many windows of entries (input columns of up to
+    2,500 entries): low, ess and the counters equal the oracle's."""
+    say("polygons")
+    b = _polygons_boundary(9000, [5, 70, 300, 1000, 2500])
+    ref = oracle.pms(b, snapshot=True)
+    low, ess, st = openph_amd.reduce(b, "pms")
+    assert np.array_equal(low, ref.low)
+    assert np.array_equal(ess, ref.ess)
+    assert st.num_iters == ref.num_iters
+    assert st.additions == ref.additions
+    assert st.merge_elems == ref.merge_elems
