@@ -1118,9 +1118,18 @@ __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
     } else if (h) {
         return;
     }
-    int p[kItems], f[kItems];
+    int p[kItems], f[kItems], lf[kItems], of[kItems], ln[kItems];
     gather_run(d.arglow, lo, -1, p);
     gather_run(d.first, lo, kIntMax, f);
+    // FUSED (short passes, bound by their chain of loads): what a mark
+    // publishes (pivdesc: the column's place and the row's left rule),
+    // loaded with the rest instead of after the scans; in the long passes
+    // the extra loads would cost more bandwidth than the latency they hide
+    if (FUSED) {
+        gather_run(d.left, lo, 0, lf);
+        load_run(reinterpret_cast<const int *>(d.off), base, d.m, 0, of);
+        load_run(d.len, base, d.m, 0, ln);
+    }
     // per slot, the largest duplicate low of the earlier tiles: FUSED, the
     // maximum over this thread's share of them (reduced over the block
     // below); else k_tilescan's prefix for this tile
@@ -1185,7 +1194,12 @@ __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
         if (cl[k] == 0 && lo[k] > c) {
             const int L = lo[k];
             marked++;
-            publish_pivot(d, L, j);
+            if (FUSED) {
+                d.arglow[L] = j;
+                d.pivdesc[L] = make_int4(j, of[k], ln[k], lf[k]);
+            } else {
+                publish_pivot(d, L, j);
+            }
             d.classes[j] = -1;
             d.clear[L] = 1;
             d.ess[L] = 0;
