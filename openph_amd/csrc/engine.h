@@ -91,6 +91,9 @@ struct Dev {
     // Phase II worklists at most this long (0 = never) are taken by
     // 256-lane teams (k_phase2_wide) instead of the 1024-entry lane groups
     int wide_max = 0;
+    // 1: the next iteration's k_first runs inside k_apply2 (k_apply2_first);
+    // the loop is then started, and resumed after a halt, by launch_first
+    int fuse_first = 0;
     // per column
     int *low = nullptr, *arglow = nullptr, *classes = nullptr, *ess = nullptr;
     int *left = nullptr, *beta = nullptr, *first = nullptr, *clear = nullptr;
@@ -142,6 +145,9 @@ void launch_set_halt(const Dev &d, int value, hipStream_t s);
 // stamps[(2 * (iter - 1) + which) % cap], unless halted.
 void launch_stamp(const Dev &d, unsigned long long *stamps, int which, int cap, hipStream_t s);
 void launch_phase1_detect(const Dev &d, hipStream_t s);
+// k_first alone: starts an iteration (Dev::fuse_first: the loop's first one,
+// or the one resumed after a halt)
+void launch_first(const Dev &d, hipStream_t s);
 void launch_phase1_mark(const Dev &d, hipStream_t s);
 void launch_apply1(const Dev &d, hipStream_t s);
 // lanes: 1 = one lane per column over all columns; 4..64 = lane groups over

@@ -888,6 +888,39 @@ void launch_set_halt(const Dev &d, int value, hipStream_t s) {
 // then issues one global atomicMin per distinct low.
 constexpr int kHash = 2 * kTile;  // open addressing, load factor <= 1/2
 
+// The first occurrences of lows for Phase I: columns base..base+kItems-1
+// with lo[k] >= 0 and no pivot for it (p[k] = arglow[lo[k]] < 0) take the
+// minimum column per low in the block's LDS table, then one atomicMin per
+// distinct low. Every thread of the block calls both, with a block barrier
+// between the init and the insert.
+__device__ __forceinline__ void first_hash_init(int *hkey, int *hval) {
+    for (int i = threadIdx.x; i < kHash; i += kBlock) {
+        hkey[i] = -1;
+        hval[i] = kIntMax;
+    }
+}
+
+__device__ __forceinline__ void first_hash(const Dev &d, int base, const int (&lo)[kItems],
+                                           const int (&p)[kItems], int *hkey, int *hval) {
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        const int lj = lo[k];
+        if (lj < 0 || p[k] > -1) continue;
+        unsigned h = ((unsigned)lj * 2654435761u) >> 21;  // 11 bits = kHash
+        while (true) {
+            const int old = atomicCAS(&hkey[h], -1, lj);
+            if (old == -1 || old == lj) break;
+            h = (h + 1) & (kHash - 1);
+        }
+        atomicMin(&hval[h], base + k);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHash; i += kBlock) {
+        const int key = hkey[i];
+        if (key > -1) atomicMin(&d.first[key], hval[i]);
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) k_first(Dev d) {
     __shared__ int hkey[kHash];
     __shared__ int hval[kHash];
@@ -925,28 +958,9 @@ __global__ void __launch_bounds__(kBlock) k_first(Dev d) {
     }
     if (blockIdx.x == 0)  // k_tile marks the members whose lows repeat
         for (int k = threadIdx.x; k < d.nmem; k += kBlock) d.mdup[k] = 0;
-    for (int i = threadIdx.x; i < kHash; i += kBlock) {
-        hkey[i] = -1;
-        hval[i] = kIntMax;
-    }
+    first_hash_init(hkey, hval);
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kItems; k++) {
-        const int lj = lo[k];
-        if (lj < 0 || p[k] > -1) continue;
-        unsigned h = ((unsigned)lj * 2654435761u) >> 21;  // 11 bits = kHash
-        while (true) {
-            const int old = atomicCAS(&hkey[h], -1, lj);
-            if (old == -1 || old == lj) break;
-            h = (h + 1) & (kHash - 1);
-        }
-        atomicMin(&hval[h], base + k);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kHash; i += kBlock) {
-        const int key = hkey[i];
-        if (key > -1) atomicMin(&d.first[key], hval[i]);
-    }
+    first_hash(d, base, lo, p, hkey, hval);
 }
 static_assert(kHash == 2048, "k_first hashes to 11 bits");
 
@@ -1219,7 +1233,7 @@ __global__ void __launch_bounds__(kBlock) k_mark(Dev d) {
 }
 
 void launch_phase1_detect(const Dev &d, hipStream_t s) {
-    k_first<<<d.ntiles, kBlock, 0, s>>>(d);
+    if (!d.fuse_first) k_first<<<d.ntiles, kBlock, 0, s>>>(d);
     k_tile<<<d.ntiles, kBlock, 0, s>>>(d);
 }
 
@@ -2585,9 +2599,82 @@ __global__ void __launch_bounds__(kBlock) k_apply2(Dev d) {
     }
 }
 
+// k_apply2 followed by the next iteration's k_first (Dev::fuse_first: the
+// queued and graph schedules without trackers or iteration times, where no
+// host step comes between two iterations), one launch less per iteration.
+// The first thread ends the iteration and, unless it halts, starts the next;
+// the blocks then take the next iteration's first occurrences from their
+// columns as just updated. The arglow read for them races with this pass's
+// publications, which is harmless: a row whose pivot is published here has
+// arglow set for Phase I, which then does not look at first[] for that row.
+// After any halt the blocks skip that part: a resumed loop (after a pool
+// compaction) starts with k_first.
+__global__ void __launch_bounds__(kBlock) k_apply2_first(Dev d) {
+    __shared__ int hkey[kHash];
+    __shared__ int hval[kHash];
+    __shared__ int s_halt;
+    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
+    int np[kItems], cr[kItems], lo[kItems], al[kItems], cl[kItems];
+    // column t's own entries, written by no other thread of the pass
+    load_run(d.newpiv, base, d.m, -1, np);
+    load_run(d.clear, base, d.m, 0, cr);
+    load_run(d.low, base, d.m, -1, lo);
+    load_run(d.arglow, base, d.m, -1, al);
+    load_run(d.classes, base, d.m, 1, cl);
+    if (threadIdx.x == 0) {
+        if (blockIdx.x == 0) {
+            iter_end(d.ctrl);
+            if (!d.ctrl->halt) iter_begin(d.ctrl);
+        }
+        s_halt = d.ctrl->halt;
+    }
+    int nl[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        const int t = base + k;
+        nl[k] = -1;
+        if (t >= d.m) continue;
+        if (np[k] > -1) {
+            publish_pivot(d, t, np[k]);
+            d.newpiv[t] = -1;
+        }
+        if (cr[k]) {
+            d.low[t] = -1;
+            d.classes[t] = 1;
+            d.clear[t] = 0;
+        }
+        const int lt = cr[k] ? -1 : lo[k];
+        if (lt > -1) d.ess[lt] = 0;
+        const int a = np[k] > -1 ? np[k] : al[k];
+        if (a > -1) {
+            d.ess[a] = 0;
+            d.ess[t] = 0;
+            d.classes[a] = -1;
+            d.classes[t] = 1;
+        }
+        // the next iteration's Phase I: unknown columns with a low
+        const int c = cr[k] || a > -1 ? 1 : cl[k];
+        nl[k] = c == 0 ? lt : -1;
+    }
+    int p[kItems];
+    gather_run(d.arglow, nl, 0, p);
+    first_hash_init(hkey, hval);
+    __syncthreads();  // s_halt and the table
+    if (s_halt) return;  // uniform over the block
+    first_hash(d, base, nl, p, hkey, hval);
+}
+
 void launch_apply2(const Dev &d, hipStream_t s) {
+    if (d.fuse_first) {
+        if (d.ntiles) k_apply2_first<<<d.ntiles, kBlock, 0, s>>>(d);
+        return;
+    }
     int gm = (d.m + kBlock - 1) / kBlock;
     k_apply2<<<gm, kBlock, 0, s>>>(d);
+}
+
+void launch_first(const Dev &d, hipStream_t s) {
+    if (d.ntiles) k_first<<<d.ntiles, kBlock, 0, s>>>(d);
 }
 
 // ---------------------------------------------------------------------------

@@ -235,6 +235,7 @@ static void enqueue_setup(const Dev &d, const WholeArgs &w, hipStream_t s) {
     if (w.pool_elems) launch_pool_cap(d, w.pool_elems, w.lcap, s);
     launch_settle(d, s);
     launch_phase0(d, s);
+    if (d.fuse_first) launch_first(d, s);
 }
 
 // The cached graph of `iters` iterations for the current buffers and shape
@@ -455,6 +456,9 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
     const bool one_read = !tracking && sched != kSchedSync;
     const bool ahead = one_read && (cfg.lanes != 0 || (same_size && c.last_lanes != 0));
     const bool iter_times = tr && tr->time_track;
+    // no host step between two iterations (no trackers, no per-iteration or
+    // per-pass times): k_apply2 also starts the next one
+    d.fuse_first = one_read && !iter_times && !cfg.timing ? 1 : 0;
     // A repeated reduction replays one graph from the set-up to the gated
     // outputs: launched one by one, the set-up's kernels leave the device
     // waiting on the host between them.
@@ -532,6 +536,7 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
     const bool timed = stats != nullptr && events && cfg.timing;
 
     if (!whole) launch_phase0(d, s);
+    if (!whole && d.fuse_first) launch_first(d, s);
     // iteration i is timed from evs[2i] to evs[2i+1] (tic/toc around the
     // loop body, pms.cu:46-103; the trackers are outside)
     std::vector<hipEvent_t> &evs = c.iter_events;
@@ -686,6 +691,7 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
             if (h.halt == kHaltCompact) {
                 if ((rc = compact(c, cfg, s, &ncompact, lcap))) return rc;
                 launch_set_halt(d, kHaltNone, s);
+                if (d.fuse_first) launch_first(d, s);  // the next iteration's start
             } else if (h.halt) {
                 return halt_status(h);
             }
@@ -797,6 +803,7 @@ int live_start(Context &c, LiveBatch &lb, const RunCfg &cfg, int cls, int max_me
     lb.grid = 2 * phase2_blocks_per_cu(lb.lanes, lb.wlds) * c.num_cu;
     lb.long_grid = c.num_cu;
     d.wide_max = lb.wlds >= 1024 && cfg.wide ? 3 * c.num_cu : 0;
+    d.fuse_first = 0;  // members join between chunks: every iteration starts with k_first
     lb.mstart.assign(1, 0);
     lb.slot_of_len.clear();
     lb.nslots = 0;
