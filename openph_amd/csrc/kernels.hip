@@ -3020,39 +3020,89 @@ void launch_compact_copy(Dev &d, hipStream_t s) {
 // ---------------------------------------------------------------------------
 
 
+// kBatchRun entries (or columns) per thread, kBlock apart (coalesced): one
+// binary search over the members for the first, then a forward step where a
+// member ends before the next (members hold thousands of entries), so that a
+// pass over the batch waits for one search per thread instead of one per
+// entry.
+constexpr int kBatchRun = 16;
+
 __global__ void k_batch_offsets(const int *__restrict__ rows1, const int *__restrict__ cols1,
                                 const int *__restrict__ noff, const int *__restrict__ moff,
                                 int count, int nnz, int *__restrict__ rows_g,
                                 int *__restrict__ cols_g, int *err) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nnz) return;
-    const int k = segment_of(noff, count, e);
-    const int base = moff[k], mk = moff[k + 1] - base;
-    const int r = rows1[e], c = cols1[e];
-    if (r < 1 || c < 1 || r > mk || c > mk) atomicOr(err, 1);
-    rows_g[e] = r + base;
-    cols_g[e] = c + base;
+    const long long e0 = (long long)blockIdx.x * kBlock * kBatchRun + threadIdx.x;
+    if (e0 >= nnz) return;
+    // the entries' loads first, all in flight with the search
+    int r[kBatchRun], c[kBatchRun];
+#pragma unroll
+    for (int i = 0; i < kBatchRun; i++) {
+        const long long el = e0 + (long long)i * kBlock;
+        const int e = el < nnz ? (int)el : nnz - 1;
+        r[i] = rows1[e];
+        c[i] = cols1[e];
+    }
+    int k = segment_of(noff, count, (int)e0);
+    int end = noff[k + 1], base = moff[k], mk = moff[k + 1] - base;
+    int bad = 0;
+#pragma unroll
+    for (int i = 0; i < kBatchRun; i++) {
+        const long long el = e0 + (long long)i * kBlock;
+        if (el >= nnz) break;
+        const int e = (int)el;
+        while (e >= end && k + 1 < count) {  // the next member (possibly empty ones first)
+            k++;
+            end = noff[k + 1];
+            base = moff[k];
+            mk = moff[k + 1] - base;
+        }
+        bad |= r[i] < 1 || c[i] < 1 || r[i] > mk || c[i] > mk;
+        rows_g[e] = r[i] + base;
+        cols_g[e] = c[i] + base;
+    }
+    if (bad) atomicOr(err, 1);
 }
 
 __global__ void k_batch_local_low(int *__restrict__ low1, const int *__restrict__ moff, int count,
                                   int m) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= m) return;
-    const int l = low1[j];
-    if (l > 0) low1[j] = l - moff[segment_of(moff, count, j)];
+    const long long j0 = (long long)blockIdx.x * kBlock * kBatchRun + threadIdx.x;
+    if (j0 >= m) return;
+    int l[kBatchRun];
+#pragma unroll
+    for (int i = 0; i < kBatchRun; i++) {
+        const long long jl = j0 + (long long)i * kBlock;
+        l[i] = low1[jl < m ? (int)jl : m - 1];
+    }
+    int k = segment_of(moff, count, (int)j0);
+    int end = moff[k + 1], base = moff[k];
+#pragma unroll
+    for (int i = 0; i < kBatchRun; i++) {
+        const long long jl = j0 + (long long)i * kBlock;
+        if (jl >= m) break;
+        const int j = (int)jl;
+        while (j >= end && k + 1 < count) {
+            k++;
+            end = moff[k + 1];
+            base = moff[k];
+        }
+        if (l[i] > 0) low1[j] = l[i] - base;
+    }
+}
+
+static int batch_run_blocks(long long n) {
+    return (int)((n + (long long)kBlock * kBatchRun - 1) / ((long long)kBlock * kBatchRun));
 }
 
 void launch_batch_offsets(const int *rows1, const int *cols1, const int *noff, const int *moff,
                           int count, int nnz, int *rows_g, int *cols_g, int *err, hipStream_t s) {
     if (nnz > 0)
-        k_batch_offsets<<<(nnz + kBlock - 1) / kBlock, kBlock, 0, s>>>(rows1, cols1, noff, moff,
-                                                                      count, nnz, rows_g, cols_g,
-                                                                      err);
+        k_batch_offsets<<<batch_run_blocks(nnz), kBlock, 0, s>>>(rows1, cols1, noff, moff, count,
+                                                                 nnz, rows_g, cols_g, err);
 }
 
 void launch_batch_local_low(int *low1, const int *moff, int count, int m, hipStream_t s) {
     if (m > 0)
-        k_batch_local_low<<<(m + kBlock - 1) / kBlock, kBlock, 0, s>>>(low1, moff, count, m);
+        k_batch_local_low<<<batch_run_blocks(m), kBlock, 0, s>>>(low1, moff, count, m);
 }
 
 }  // namespace oph
