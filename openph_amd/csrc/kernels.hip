@@ -397,23 +397,24 @@ __device__ __forceinline__ void wave_lower_bounds(const int *__restrict__ cols1,
 // whole (a = 0, e0 = 0, nothing in the pool): column j's segment holds
 // len + slot0 entries, so its offset is its COO start + j * slot0 and the
 // layout needs no scan; its length is marked present in slotmap.
+template <int CPW>
 __global__ void __launch_bounds__(kBlock) k_init_cols(Dev d, const int *__restrict__ rows1,
                                                       const int *__restrict__ cols1, int a,
                                                       int e0, int e1, int whole) {
     constexpr int NW = kBlock / 64;
-    constexpr int U = 4;  // windows of 64 entries read at once
-    __shared__ int s_start[NW][65];
+    constexpr int U = 4;        // windows of 64 entries read at once
+    constexpr int Q = CPW / 64;  // columns per lane
+    __shared__ int s_start[NW][CPW + 1];
     __shared__ int s_max;
     const int lane = lane_id(), wid = threadIdx.x >> 6;
-    const long long jw = a + (((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64;
-    const int key0 = jw < d.m ? (int)jw + 1 : d.m + 1;       // the wave's first column, 1-based
-    const int keyE = jw + 64 < d.m ? (int)jw + 65 : d.m + 1;  // one past its last
+    const long long jw = a + (((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * CPW;
+    const int key0 = jw < d.m ? (int)jw + 1 : d.m + 1;             // the wave's first column, 1-based
+    const int keyE = jw + CPW < d.m ? (int)jw + CPW + 1 : d.m + 1;  // one past its last
     if (threadIdx.x == 0) s_max = 0;
     int lo0, loE;
     wave_lower_bounds(cols1, e0, e1, key0, keyE, lo0, loE);
     if (loE < lo0) loE = lo0;  // unsorted input; flagged by k_entries
-    s_start[wid][lane] = loE;
-    if (lane == 0) s_start[wid][64] = loE;
+    for (int i = lane; i <= CPW; i += 64) s_start[wid][i] = loE;
     // column c1 > c0 at entry e: the columns (c0, c1] of the wave start at e
     const int nwin = (loE - lo0 + 63) >> 6;
     for (int w0 = 0; w0 < nwin; w0 += U) {
@@ -436,49 +437,79 @@ __global__ void __launch_bounds__(kBlock) k_init_cols(Dev d, const int *__restri
         }
     }
     __syncthreads();
-    const long long jl = jw + lane;
-    const bool live = jl < d.m;
-    const int lo = s_start[wid][lane];
-    int hi = s_start[wid][lane + 1];
-    if (hi < lo) hi = lo;  // unsorted input; flagged by k_entries
-    const int n = live ? hi - lo : 0;
-    int wmax = n;
+    int lo[Q], hi[Q], n[Q], l[Q];
+    int wmax = 0;
+    unsigned long long present = 0;  // lengths below 64 of the lane's columns
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const long long jl = jw + q * 64 + lane;
+        lo[q] = s_start[wid][q * 64 + lane];
+        hi[q] = s_start[wid][q * 64 + lane + 1];
+        if (hi[q] < lo[q]) hi[q] = lo[q];  // unsorted input; flagged by k_entries
+        n[q] = jl < d.m ? hi[q] - lo[q] : 0;
+        wmax = n[q] > wmax ? n[q] : wmax;
+        if (jl < d.m && n[q] < 64) present |= 1ull << n[q];
+        // the last row: the initial low (compute_low, kernels.cu:40-52)
+        l[q] = n[q] ? rows1[hi[q] - 1] - 1 : -1;
+        if (l[q] < -1 || l[q] >= d.m) l[q] = -1;  // out of range; flagged by k_entries
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         const int x = __shfl_xor(wmax, o, 64);
         wmax = x > wmax ? x : wmax;
+        present |= __shfl_xor(present, o, 64);
     }
     if (lane == 0 && wmax) atomicMax(&s_max, wmax);
+    // the lengths present, one store per length and wave instead of one per
+    // column (all columns of a complex share a few lengths)
+    if (whole && lane < 64 && (present >> lane & 1) && lane <= d.m) d.slotmap[lane] = 1;
     __syncthreads();
     if (threadIdx.x == 0 && s_max) atomicMax(&d.ctrl->maxlen, s_max);
-    if (!live) return;
-    const int j = (int)jl;
-    d.off2[j] = (unsigned)lo;
-    d.len[j] = n;
-    if (whole) {
-        d.off[j] = (unsigned)((unsigned long long)lo + (unsigned long long)j * d.slot0);
-        d.cap[j] = n + d.slot0;
-        if (n <= d.m) d.slotmap[n] = 1;  // (longer only in malformed input)
-        if (j == 0)
-            d.ctrl->pool_top = (unsigned long long)e1 + (unsigned long long)d.m * d.slot0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const long long jl = jw + q * 64 + lane;
+        if (jl >= d.m) break;
+        const int j = (int)jl;
+        d.off2[j] = (unsigned)lo[q];
+        d.len[j] = n[q];
+        if (whole) {
+            d.off[j] = (unsigned)((unsigned long long)lo[q] + (unsigned long long)j * d.slot0);
+            d.cap[j] = n[q] + d.slot0;
+            if (n[q] >= 64 && n[q] <= d.m) d.slotmap[n[q]] = 1;  // (beyond m only in malformed input)
+            if (j == 0)
+                d.ctrl->pool_top = (unsigned long long)e1 + (unsigned long long)d.m * d.slot0;
+        }
+        d.low[j] = l[q];
+        d.arglow[j] = -1;
+        d.classes[j] = 0;
+        d.ess[j] = 1;
+        d.left[j] = kIntMax;
+        d.beta[j] = -1;
+        d.first[j] = kIntMax;
+        d.clear[j] = 0;
+        d.newpiv[j] = -1;
+        d.pivdesc[j] = make_int4(-1, 0, 0, 0);
     }
-    int l = n ? rows1[hi - 1] - 1 : -1;
-    if (l < -1 || l >= d.m) l = -1;  // out of range; flagged by k_entries
-    d.low[j] = l;
-    d.arglow[j] = -1;
-    d.classes[j] = 0;
-    d.ess[j] = 1;
-    d.left[j] = kIntMax;
-    d.beta[j] = -1;
-    d.first[j] = kIntMax;
-    d.clear[j] = 0;
-    d.newpiv[j] = -1;
-    d.pivdesc[j] = make_int4(-1, 0, 0, 0);
 }
 
-static int init_cols_blocks(int m) {
-    const long long waves = ((long long)m + 63) / 64;
+// Columns per wave: 256 for large matrices (one search per 256 columns; 4x
+// fewer waves), 64 below a million columns (more waves in flight).
+constexpr int kInitColsLarge = 1 << 20;
+
+static int init_cols_blocks(int m, int cpw) {
+    const long long waves = ((long long)m + cpw - 1) / cpw;
     return (int)((waves * 64 + kBlock - 1) / kBlock);
+}
+
+static void launch_init_cols(const Dev &d, const int *rows1, const int *cols1, int a, int e0,
+                             int e1, int whole, hipStream_t s) {
+    const int n = d.m - a;
+    if (n >= kInitColsLarge)
+        k_init_cols<256><<<init_cols_blocks(n, 256), kBlock, 0, s>>>(d, rows1, cols1, a, e0, e1,
+                                                                      whole);
+    else
+        k_init_cols<64><<<init_cols_blocks(n, 64), kBlock, 0, s>>>(d, rows1, cols1, a, e0, e1,
+                                                                    whole);
 }
 
 // Validation, pool copy (0-based) and left[] by atomicMin. A row occurs in
@@ -584,8 +615,7 @@ __global__ void k_beta(Dev d, int a) {
 }
 
 void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t s) {
-    if (d.m > 0)
-        k_init_cols<<<init_cols_blocks(d.m), kBlock, 0, s>>>(d, rows1, cols1, 0, 0, d.nnz, 1);
+    if (d.m > 0) launch_init_cols(d, rows1, cols1, 0, 0, d.nnz, 1, s);
     if (d.nnz > 0)
         k_entries<<<entries_blocks(d.nnz), kEntriesBlock, 0, s>>>(d, rows1, cols1, 0, 0, d.nnz);
 }
@@ -647,7 +677,7 @@ void launch_setup_range(const Dev &d, int a, int e0, int e1, const int *rows1, c
                         hipStream_t s) {
     const int ge = entries_blocks((long long)e1 - e0);
     if (d.m > a)
-        k_init_cols<<<init_cols_blocks(d.m - a), kBlock, 0, s>>>(d, rows1, cols1, a, e0, e1, 0);
+        launch_init_cols(d, rows1, cols1, a, e0, e1, 0, s);
     launch_scan(d, 0, d.off, s, a, a > 0);
     if (e1 > e0) k_entries<<<ge, kEntriesBlock, 0, s>>>(d, rows1, cols1, a, e0, e1);
 }
