@@ -516,22 +516,23 @@ static void launch_init_cols(const Dev &d, const int *rows1, const int *cols1, i
 // many columns (an edge of a Vietoris-Rips complex in every triangle on it,
 // a few hundred columns apart), and device-scope atomics run at some 20 G/s
 // for the whole GPU, slower still on one address. So a block takes
-// kEntriesTile consecutive entries, the minimum column per row first in an
+// 1,024 x PER consecutive entries, the minimum column per row first in an
 // LDS hash table, and issues one atomic per distinct row: 219 K atomics
-// instead of 302 K (1,024-entry tiles) on Torus-100, 12.9 M instead of
-// 22.8 M on a batch of 512 VR 2-skeleta. Larger tiles cut the atomics
-// further, but their fewer blocks take longer than the atomics they save
-// (measured at 8 and 16 entries per thread). A row that finds no slot within
-// kProbe probes (cubical grids: few repeats, many distinct rows) takes its
-// atomic at once.
+// instead of 302 K (1,024-entry tiles) on Torus-100. Larger tiles cut the
+// atomics further (4.6 M instead of 22.8 M on a batch of 512 VR 2-skeleta
+// with 16,384-entry tiles), but on a small matrix their fewer blocks take
+// longer than the atomics they save (Torus-100: 31 blocks, 38.5 against
+// 31 us), so the tile grows with the input (PER entries per thread). A row
+// that finds no slot within kProbe probes (cubical grids: few repeats, many
+// distinct rows) takes its atomic at once.
 constexpr int kEntriesBlock = 1024;
-constexpr int kEntriesPer = 4;  // entries per thread
-constexpr int kEntriesTile = kEntriesBlock * kEntriesPer;
 constexpr int kEntriesHash = 8192;  // slots (key and value: 64 KB of LDS)
 constexpr int kProbe = 8;
+constexpr long long kEntriesLarge = 4 << 20;  // entries from which PER = 16
 
 // Entries [e0, e1), of columns [a, d.m): a row below a (another matrix of
 // a batch) is out of range.
+template <int PER>
 __global__ void __launch_bounds__(kEntriesBlock) k_entries(Dev d, const int *__restrict__ rows1,
                                                           const int *__restrict__ cols1, int a,
                                                           int e0, int e1) {
@@ -545,8 +546,8 @@ __global__ void __launch_bounds__(kEntriesBlock) k_entries(Dev d, const int *__r
     }
     __syncthreads();
     int bad = 0;
-    const long long tile0 = (long long)e0 + (long long)blockIdx.x * kEntriesTile;
-    for (int k0 = 0; k0 < kEntriesPer; k0 += R) {
+    const long long tile0 = (long long)e0 + (long long)blockIdx.x * (kEntriesBlock * PER);
+    for (int k0 = 0; k0 < PER; k0 += R) {
         int c[R], r[R], b[R];
 #pragma unroll
         for (int u = 0; u < R; u++) {
@@ -602,8 +603,17 @@ __global__ void __launch_bounds__(kEntriesBlock) k_entries(Dev d, const int *__r
 }
 static_assert(kEntriesHash == 1 << 13, "k_entries hashes to 13 bits");
 
-static int entries_blocks(long long n) {
-    return (int)((n + kEntriesTile - 1) / kEntriesTile);
+static void launch_entries(const Dev &d, const int *rows1, const int *cols1, int a, int e0,
+                           int e1, hipStream_t s) {
+    const long long n = (long long)e1 - e0;
+    if (n <= 0) return;
+    if (n >= kEntriesLarge) {
+        constexpr long long T = kEntriesBlock * 16;
+        k_entries<16><<<(int)((n + T - 1) / T), kEntriesBlock, 0, s>>>(d, rows1, cols1, a, e0, e1);
+    } else {
+        constexpr long long T = kEntriesBlock * 4;
+        k_entries<4><<<(int)((n + T - 1) / T), kEntriesBlock, 0, s>>>(d, rows1, cols1, a, e0, e1);
+    }
 }
 
 __global__ void k_beta(Dev d, int a) {
@@ -616,8 +626,7 @@ __global__ void k_beta(Dev d, int a) {
 
 void launch_setup(const Dev &d, const int *rows1, const int *cols1, hipStream_t s) {
     if (d.m > 0) launch_init_cols(d, rows1, cols1, 0, 0, d.nnz, 1, s);
-    if (d.nnz > 0)
-        k_entries<<<entries_blocks(d.nnz), kEntriesBlock, 0, s>>>(d, rows1, cols1, 0, 0, d.nnz);
+    launch_entries(d, rows1, cols1, 0, 0, d.nnz, s);
 }
 
 // After the set-up of the whole matrix, one pass:
@@ -675,11 +684,10 @@ void launch_settle(const Dev &d, hipStream_t s) {
 
 void launch_setup_range(const Dev &d, int a, int e0, int e1, const int *rows1, const int *cols1,
                         hipStream_t s) {
-    const int ge = entries_blocks((long long)e1 - e0);
     if (d.m > a)
         launch_init_cols(d, rows1, cols1, a, e0, e1, 0, s);
     launch_scan(d, 0, d.off, s, a, a > 0);
-    if (e1 > e0) k_entries<<<ge, kEntriesBlock, 0, s>>>(d, rows1, cols1, a, e0, e1);
+    launch_entries(d, rows1, cols1, a, e0, e1, s);
 }
 
 __global__ void k_assign_slot(Dev d, int a) {
