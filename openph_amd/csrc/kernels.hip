@@ -1589,12 +1589,16 @@ struct WaveTeam {
     }
 };
 
+// The scan and the broadcast take one block barrier each: they alternate
+// between two LDS buffers, and a buffer is written again only after the
+// next call's barrier, which every wave reaches once done reading it.
 template <int BT>
 struct BlockTeam {
     static constexpr int size = BT;
     static constexpr bool wave = false;
-    int *s_scan;                 // BT / 64 ints of LDS
-    unsigned long long *s_word;  // one word of LDS
+    int *s_scan;                 // 2 * BT / 64 ints of LDS
+    unsigned long long *s_word;  // two words of LDS
+    mutable int ps = 0, pb = 0;  // the buffers' turns (the same on every thread)
     __device__ int lane() const { return threadIdx.x; }
     __device__ void sync() const { __syncthreads(); }
     __device__ void fence() const {
@@ -1602,14 +1606,29 @@ struct BlockTeam {
         __syncthreads();
     }
     __device__ int scan(int v, int &total) const {
-        return block_exclusive_scan(v, SumOp<int>(), 0, s_scan, &total);
+        int *lds = s_scan + ps * (BT / 64);
+        ps ^= 1;
+        const int lane = lane_id(), wid = threadIdx.x >> 6;
+        const int incl = wave_inclusive_scan(v, SumOp<int>(), 0);
+        const int excl = dpp_or<0x138, 0xF>(0, incl);  // wave_shr:1
+        if (lane == 63) lds[wid] = incl;
+        __syncthreads();
+        int pre = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < BT / 64; w++) {
+            const int x = lds[w];
+            pre += w < wid ? x : 0;
+            tot += x;
+        }
+        total = tot;
+        return pre + excl;
     }
     __device__ unsigned long long bcast(unsigned long long x) const {
-        if (threadIdx.x == 0) *s_word = x;
+        unsigned long long *w = s_word + pb;
+        pb ^= 1;
+        if (threadIdx.x == 0) *w = x;
         __syncthreads();
-        const unsigned long long v = *s_word;
-        __syncthreads();
-        return v;
+        return *w;
     }
 };
 
@@ -2057,9 +2076,9 @@ constexpr int kLongTeam = 256;
 
 __global__ void __launch_bounds__(kLongTeam) k_phase2_long(Dev d) {
     __shared__ int lds[3 * kLongW];
-    __shared__ int s_scan[kLongTeam / 64];
-    __shared__ unsigned long long s_word;
-    const BlockTeam<kLongTeam> t{s_scan, &s_word};
+    __shared__ int s_scan[2 * kLongTeam / 64];
+    __shared__ unsigned long long s_word[2];
+    const BlockTeam<kLongTeam> t{s_scan, s_word};
     if (d.ctrl->halt) return;
     const int nlong = d.ctrl->nlong;
     Phase2Counters ctr;
@@ -2079,9 +2098,9 @@ __global__ void __launch_bounds__(kLongTeam) k_phase2_long(Dev d) {
 // get shorter steps.
 __global__ void __launch_bounds__(kLongTeam) k_phase2_wide(Dev d) {
     __shared__ int lds[3 * kLongW];
-    __shared__ int s_scan[kLongTeam / 64];
-    __shared__ unsigned long long s_word;
-    const BlockTeam<kLongTeam> t{s_scan, &s_word};
+    __shared__ int s_scan[2 * kLongTeam / 64];
+    __shared__ unsigned long long s_word[2];
+    const BlockTeam<kLongTeam> t{s_scan, s_word};
     if (d.ctrl->halt) return;
     const int nwork = d.ctrl->nwork;
     if (nwork > d.wide_max) return;
@@ -2225,9 +2244,9 @@ __global__ void __launch_bounds__(kLongBlock) k_serial(Dev d, const int *__restr
                                                        unsigned long long *start) {
     constexpr int W = kLongW;
     __shared__ int lds[3 * W];
-    __shared__ int s_scan[kLongBlock / 64];
-    __shared__ unsigned long long s_word;
-    const BlockTeam<kLongBlock> t{s_scan, &s_word};
+    __shared__ int s_scan[2 * kLongBlock / 64];
+    __shared__ unsigned long long s_word[2];
+    const BlockTeam<kLongBlock> t{s_scan, s_word};
     const int gl = threadIdx.x;
     Phase2Counters ctr;
     if (start && gl == 0) *start = (unsigned long long)wall_clock64();
@@ -2436,10 +2455,10 @@ __global__ void __launch_bounds__(kLongBlock) k_ph_row(Dev d, RowLists r, int al
                                                       unsigned long long *start) {
     constexpr int W = kLongW;
     __shared__ int lds[3 * W];
-    __shared__ int s_scan[kLongBlock / 64];
-    __shared__ unsigned long long s_word;
+    __shared__ int s_scan[2 * kLongBlock / 64];
+    __shared__ unsigned long long s_word[2];
     __shared__ int s_n;
-    const BlockTeam<kLongBlock> t{s_scan, &s_word};
+    const BlockTeam<kLongBlock> t{s_scan, s_word};
     const int gl = threadIdx.x;
     int *sP = lds, *sB = lds + W, *sC = lds + 2 * W;
     Phase2Counters ctr;
