@@ -1660,8 +1660,9 @@ __device__ int team_xor_lds(const Team &t, int *a, int na, const int *b, int nb,
     const int d1 = d0 + K < n ? d0 + K : n;
     const int cnt = mp_walk_lds<false>(a, na, b, nb, d0, d1, c + d0);
     int total;
-    const int ex = t.scan(cnt, total);  // every lane's walk is over after this
-    t.sync();
+    // every lane's walk is over after the scan (a block barrier for a
+    // block team, the lanes' own wave for a wave team)
+    const int ex = t.scan(cnt, total);
     // four reads, then four writes: one LDS round trip per four survivors
     for (int q = 0; q < cnt; q += 4) {
         const int x0 = c[d0 + q];
@@ -1680,8 +1681,9 @@ __device__ int team_xor_lds(const Team &t, int *a, int na, const int *b, int nb,
 // can issue global loads between them. xor_walk: survivors go to provisional
 // slots of c at their first diagonal, the team scan gives their places;
 // returns the sum's length, with the lane's survivor count, first diagonal
-// and output place in w. xor_copy (after a team barrier: every walk is over):
-// the survivors to their places in b, followed by a new sentinel. a may be in
+// and output place in w. xor_copy (after xor_walk's scan every walk is over,
+// so b is free to write): the survivors to their places in b, followed by a
+// new sentinel. a may be in
 // LDS (LDS_A, with its sentinel) or in the pool; b[nb] holds the sentinel
 // kIntMax, so n + 1 entries of b's buffer are used.
 struct XorWalk {
@@ -1725,8 +1727,8 @@ __device__ __forceinline__ void xor_copy(const XorWalk &w, int *b, int *c, int s
 template <bool LDS_A, class Team>
 __device__ int team_xor_into_b(const Team &t, const int *a, int na, int *b, int nb, int *c,
                                int spare) {
+    // every walk is over after xor_walk's scan: b can be written
     const XorWalk w = xor_walk<LDS_A>(t, a, na, b, nb, c);
-    t.sync();
     xor_copy(w, b, c, spare);
     return w.total;
 }
@@ -1880,7 +1882,7 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
                     }
                     have = true;
                 }
-                t.sync();
+                // every walk is over after xor_walk's scan
                 xor_copy(w, sB, sC, W - 1);
                 // Stores go after the loads they would otherwise delay: a
                 // wait for a load also waits for every older store. Every
