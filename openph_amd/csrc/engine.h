@@ -45,6 +45,7 @@ struct Ctrl {
     int stalls;                    // consecutive overflowing passes without progress
     int prev_marks;                // marks at the end of the previous iteration
     int overflows;                 // Phase II passes cut short by a full pool
+    int suspended;                 // this iteration's Phase II was cut short and resumed
     unsigned long long prev_add;   // additions at the end of the previous iteration
     unsigned long long pool_peak;  // largest pool_top seen at an iteration end
     // trackers of one iteration (tracker.cu:92-134), as exact integers
@@ -71,6 +72,7 @@ enum : int {
     kHaltIters = 4,      // iteration cap reached
     kHaltNoMem = 5,      // two overflowing passes in a row made no progress
     kHaltError = 6,      // err bits set
+    kHaltSuspend = 7,    // Phase II ran out of pool: compact, finish the same pass, end the iteration
 };
 
 // Validation error bits (Ctrl::err).
@@ -141,6 +143,9 @@ void launch_phase0(const Dev &d, hipStream_t s);
 // flag before they resume.
 void launch_iter_begin(const Dev &d, hipStream_t s);
 void launch_set_halt(const Dev &d, int value, hipStream_t s);
+// Before a suspended Phase II pass runs again (kHaltSuspend, after the
+// compaction): clears the halt, the overflow flag and the long-column list.
+void launch_resume2(const Dev &d, hipStream_t s);
 // Device clock (wall_clock64) of the current iteration into
 // stamps[(2 * (iter - 1) + which) % cap], unless halted.
 void launch_stamp(const Dev &d, unsigned long long *stamps, int which, int cap, hipStream_t s);

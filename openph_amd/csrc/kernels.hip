@@ -188,7 +188,7 @@ __device__ __forceinline__ unsigned slot_size(int len, int slot0) {
 //   mode 1: compaction, size = len for published pivots (read, never written
 //           again), 0 for other cleared columns (never read again),
 //           max(len, slot0) else. Whether a column is a published pivot
-//           comes from arglow (work[j] = 1 + the row it is the pivot of,
+//           comes from arglow (longw[j] = 1 + the row it is the pivot of,
 //           written by k_pivot_rows), not from classes: an algorithm may
 //           mark a pivot's column positive, and its data must survive.
 // Offsets go to `out`; the total to scan_tmp[ntiles] and ctrl->pool_top.
@@ -198,7 +198,7 @@ __device__ __forceinline__ unsigned col_size(const Dev &d, int j, int mode) {
     if (j >= d.m) return 0u;
     int n = d.len[j];
     if (mode == 0) return slot_size(n, d.slot0);
-    if (d.work[j]) return (unsigned)n;
+    if (d.longw[j]) return (unsigned)n;
     if (d.classes[j] == 1) return 0u;
     return slot_size(n, d.slot0);
 }
@@ -300,6 +300,7 @@ __global__ void k_ctrl_init(Ctrl *c, unsigned long long cap, int max_iters, int 
         c->stalls = 0;
         c->prev_marks = 0;
         c->overflows = 0;
+        c->suspended = 0;
         c->prev_add = 0;
         c->pool_peak = 0;
         for (int k = 0; k < kStripes; k++) {
@@ -839,6 +840,7 @@ __device__ __forceinline__ void iter_begin(Ctrl *c) {
     c->iter += 1;
     c->any_dup = 0;
     c->overflow = 0;
+    c->suspended = 0;
     c->nwork = 0;
     c->nlong = 0;
 }
@@ -865,7 +867,7 @@ __device__ void iter_end(Ctrl *c) {
     }
     const bool progress = c->additions != c->prev_add || c->marks != c->prev_marks;
     if (c->pool_top > c->pool_peak) c->pool_peak = c->pool_top;
-    if (!c->overflow && !progress && c->any_dup) {
+    if (!c->overflow && !progress && c->any_dup && !c->suspended) {
         c->halt = kHaltStuck;
         return;
     }
@@ -879,6 +881,45 @@ __device__ void iter_end(Ctrl *c) {
     if (c->overflow) c->overflows += 1;
     if (c->overflow || c->pool_top * 4 > c->pool_cap * 3) c->halt = kHaltCompact;
 }
+
+// A Phase II pass that ran out of pool is suspended instead of ended: the
+// host compacts the pool and runs the same pass again over the same worklist
+// (each column goes on from the state it committed; the pivots it reads are
+// still those of the iteration's start, since k_apply2 published nothing),
+// then k_apply2 ends the iteration. The iteration then does what it would
+// have done with a larger pool: the same additions, the same iteration count.
+// Two suspensions in a row without progress mean the pool budget cannot hold
+// the work. Run by one thread of k_apply2, which otherwise does nothing while
+// Ctrl::overflow is set.
+__device__ void iter_suspend(Ctrl *c) {
+    fold_stripes(c);
+    if (c->halt) return;
+    if (c->err) {
+        c->halt = kHaltError;
+        return;
+    }
+    const bool progress = c->additions != c->prev_add || c->marks != c->prev_marks;
+    c->stalls = progress ? 0 : c->stalls + 1;
+    if (c->stalls >= 2) {
+        c->halt = kHaltNoMem;
+        return;
+    }
+    c->prev_add = c->additions;
+    c->prev_marks = c->marks;
+    c->overflows += 1;
+    c->suspended = 1;
+    if (c->pool_top > c->pool_peak) c->pool_peak = c->pool_top;
+    c->halt = kHaltSuspend;
+}
+
+__global__ void k_resume2(Ctrl *c) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    c->halt = kHaltNone;
+    c->overflow = 0;
+    c->nlong = 0;
+}
+
+void launch_resume2(const Dev &d, hipStream_t s) { k_resume2<<<1, 64, 0, s>>>(d.ctrl); }
 
 __global__ void k_set_halt(Ctrl *c, int value) {
     if (threadIdx.x == 0 && blockIdx.x == 0) c->halt = value;
@@ -1800,7 +1841,10 @@ __device__ void phase2_column(const Team &t, const Dev &d, int j, int *buf, Phas
     const int gl = t.lane();
     constexpr int G = Team::size;
     int lj = d.low[j];
-    int4 pd = d.pivdesc[lj];
+    // a pass that runs again after a suspension meets columns whose chain
+    // already ended, with no low
+    int4 pd = d.pivdesc[lj > -1 ? lj : 0];
+    if (lj < 0) pd.x = -1;
     unsigned coff = d.off[j];
     int clen = d.len[j], ccap = d.cap[j];
     int *sA = buf, *sB = buf + WA, *sC = buf + WA + W;
@@ -2631,6 +2675,12 @@ void launch_ess_at_lows(const Dev &d, hipStream_t s) {
 // after a halted iteration nothing is pending (newpiv and clear were reset
 // by the last pass) and the pass changes nothing.
 __global__ void __launch_bounds__(kBlock) k_apply2(Dev d) {
+    // a Phase II pass cut short by a full pool publishes nothing until it
+    // has run again after the compaction (iter_suspend)
+    if (d.ctrl->overflow) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) iter_suspend(d.ctrl);
+        return;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) iter_end(d.ctrl);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < d.m) {
@@ -2674,12 +2724,21 @@ __global__ void __launch_bounds__(kBlock) k_apply2_first(Dev d) {
     __shared__ int s_halt;
     const int base = blockIdx.x * kTile + threadIdx.x * kItems;
     int np[kItems], cr[kItems], lo[kItems], al[kItems], cl[kItems];
+    // set by Phase II only, so every block reads the same value: block 0
+    // resets it only when it was clear
+    const int ovf = d.ctrl->overflow;
     // column t's own entries, written by no other thread of the pass
     load_run(d.newpiv, base, d.m, -1, np);
     load_run(d.clear, base, d.m, 0, cr);
     load_run(d.low, base, d.m, -1, lo);
     load_run(d.arglow, base, d.m, -1, al);
     load_run(d.classes, base, d.m, 1, cl);
+    // a Phase II pass cut short by a full pool publishes nothing until it
+    // has run again after the compaction (iter_suspend)
+    if (ovf) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) iter_suspend(d.ctrl);
+        return;
+    }
     if (threadIdx.x == 0) {
         if (blockIdx.x == 0) {
             iter_end(d.ctrl);
@@ -3019,18 +3078,20 @@ void launch_clear_stamps(const Dev &d, hipStream_t s) {
 // pool; cleared columns are dropped (never reduced again, never a pivot).
 // ---------------------------------------------------------------------------
 
-// work[] (free between iterations: k_apply1 rebuilds the worklist) marks the
-// published pivots for the compaction: work[p] = 1 + i where arglow[i] = p.
+// longw[] (free outside Phase II: the pass rebuilds the long-column list,
+// also when it runs again after a suspension, which needs work[] intact)
+// marks the published pivots for the compaction: longw[p] = 1 + i where
+// arglow[i] = p.
 __global__ void k_pivot_rows_clear(Dev d) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < d.m) d.work[j] = 0;
+    if (j < d.m) d.longw[j] = 0;
 }
 
 __global__ void k_pivot_rows(Dev d) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.m) return;
     const int p = d.arglow[i];
-    if (p > -1 && p < d.m) d.work[p] = i + 1;
+    if (p > -1 && p < d.m) d.longw[p] = i + 1;
 }
 
 __global__ void k_copy_cols(Dev d) {
@@ -3048,7 +3109,7 @@ __global__ void k_copy_cols(Dev d) {
     int *dst = d.pool2 + no;
     for (int i = 0; i < n; i++) dst[i] = src[i];
     d.cap[j] = (int)sz;
-    const int r = d.work[j] - 1;  // the row whose pivot j is, or -1
+    const int r = d.longw[j] - 1;  // the row whose pivot j is, or -1
     if (r > -1) d.pivdesc[r] = make_int4(j, (int)no, n, d.left[r]);
 }
 

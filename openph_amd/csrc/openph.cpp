@@ -149,11 +149,29 @@ static int read_ctrl(Context &c, hipStream_t s) {
     return OPENPH_OK;
 }
 
+// A Phase II pass cut short by a full pool (kHaltSuspend): the pool is
+// compacted, the same pass runs again over the same worklist (each column
+// goes on from the state it committed, against the pivots of the
+// iteration's start), and k_apply2 then ends the iteration, so that the
+// iteration does what it would have done with a larger pool.
+static int resume_suspended(Context &c, const RunCfg &cfg, int lanes, int wlds, int grid,
+                            int long_grid, hipStream_t s, int *ncompact, size_t &lcap) {
+    int rc = compact(c, cfg, s, ncompact, lcap);
+    if (rc) return rc;
+    Dev &d = c.d;
+    launch_resume2(d, s);
+    launch_phase2(d, lanes, wlds, grid, long_grid, s);
+    launch_apply2(d, s);
+    if (d.ltrue) launch_track_iter(d, s);  // the finished iteration's trackers
+    return OPENPH_OK;
+}
+
 static int halt_status(const Ctrl &h) {
     switch (h.halt) {
         case kHaltStuck: return OPENPH_EITER;
         case kHaltIters: return OPENPH_EITER;
         case kHaltNoMem: return OPENPH_ENOMEM;
+        case kHaltSuspend: return OPENPH_EITER;  // always resumed before this is asked
         case kHaltError: {
             int rc = err_from_bits(h.err);
             return rc ? rc : OPENPH_EINVAL;
@@ -585,6 +603,19 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
                 return rc;
             }
             reads++;
+            if (h.halt == kHaltSuspend) {
+                while (h.halt == kHaltSuspend) {
+                    if ((rc = resume_suspended(c, cfg, g.lanes, g.wlds, g.grid, g.long_grid, s,
+                                               &ncompact, lcap)))
+                        return rc;
+                    if (iter_times && iter <= m)
+                        OPH_CHECK(hipEventRecord(evs[2 * iter + 1], s));
+                    if ((rc = read_ctrl(c, s))) return rc;
+                    reads++;
+                }
+                // the trackers of the finished iteration
+                if (tracking && (rc = track_state(c, tr, iter, s, m, ess_true))) return rc;
+            }
             if (h.halt == kHaltCompact) {
                 if ((rc = compact(c, cfg, s, &ncompact, lcap))) return rc;
                 launch_set_halt(d, kHaltNone, s);
@@ -647,6 +678,21 @@ int run_pms(Context &c, const int *rows1, const int *cols1, int m, int nnz, int 
             if ((rc = read_ctrl(c, s))) return rc;
             reads++;
             if (ahead && iter0 == 0) lcap = h.pool_cap;  // set by the set-up
+            while (h.halt == kHaltSuspend) {
+                if ((rc = resume_suspended(c, cfg, sh.lanes, sh.wlds, sh.grid, sh.long_grid, s,
+                                           &ncompact, lcap)))
+                    return rc;
+                if (d.trk_ring)
+                    OPH_CHECK(hipMemcpyAsync(c.h_ring, d.trk_ring,
+                                             4 * kIterRing * sizeof(unsigned long long),
+                                             hipMemcpyDeviceToHost, s));
+                if (d.its)
+                    OPH_CHECK(hipMemcpyAsync(c.h_its, d.its,
+                                             2 * kIterRing * sizeof(unsigned long long),
+                                             hipMemcpyDeviceToHost, s));
+                if ((rc = read_ctrl(c, s))) return rc;
+                reads++;
+            }
             // iterations of this chunk that reached Phase II: all that began,
             // but the one that found convergence (or the cap) at its start
             const int begun = h.iter - iter0;
@@ -902,13 +948,21 @@ int live_chunk_finish(Context &c, LiveBatch &lb, const RunCfg &cfg, hipStream_t 
     Dev &d = c.d;
     OPH_CHECK(hipStreamSynchronize(s));
     const Ctrl &h = *c.h_ctrl;
-    if (h.halt == kHaltCompact) {
+    if (h.halt == kHaltCompact || h.halt == kHaltSuspend) {
         RunCfg cc = cfg;
         cc.pool_max = std::min(cfg.pool_max, c.cap_pool2 ? std::max(c.cap_pool, c.cap_pool2)
                                                          : c.cap_pool);
-        int rc = compact(c, cc, s, &lb.compactions, lb.lcap);
+        int rc;
+        if (h.halt == kHaltSuspend) {
+            // the rest of the iteration follows on the stream; a pass that
+            // runs out again is caught at the end of the next chunk
+            rc = resume_suspended(c, cc, lb.lanes, lb.wlds, lb.grid, lb.long_grid, s,
+                                  &lb.compactions, lb.lcap);
+        } else {
+            rc = compact(c, cc, s, &lb.compactions, lb.lcap);
+            launch_set_halt(d, kHaltNone, s);
+        }
         if (rc) return rc;
-        launch_set_halt(d, kHaltNone, s);
         c.h_ctrl->halt = kHaltNone;
         return OPENPH_OK;
     }

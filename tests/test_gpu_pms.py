@@ -178,6 +178,10 @@ def test_small_pool_compacts_and_stays_exact():
     assert st.compactions > 0 and st.overflows > 0
     assert np.array_equal(low, f["low"])
     assert np.array_equal(ess, f["ess"])
+    # suspended passes run again before their iteration ends
+    snap = INDEX["torus100"]["pms_snapshot"]
+    assert st.additions == snap["additions"]
+    assert st.num_iters == snap["num_iters"]
 
 
 def test_small_pool_with_handoffs_compacts_and_stays_exact():
@@ -195,6 +199,42 @@ def test_small_pool_with_handoffs_compacts_and_stays_exact():
     assert st.compactions > 0
     assert np.array_equal(low, ref.low)
     assert np.array_equal(ess, ref.ess)
+    assert st.additions == ref.additions
+    assert st.num_iters == ref.num_iters
+
+
+_SUSPEND = {}
+
+
+def _suspend_case():
+    if not _SUSPEND:
+        b = C.cubical_lower_star(C.smoothed_field(np.random.default_rng(11), 43))
+        _SUSPEND["b"] = b
+        _SUSPEND["ref"] = oracle.pms(b, snapshot=True)
+    return _SUSPEND["b"], _SUSPEND["ref"]
+
+
+@pytest.mark.parametrize("schedule", [1, 2, 3])
+@pytest.mark.parametrize("lanes,wlds", [(0, 0), (8, 32), (32, 256), (1, 0)])
+def test_phase2_out_of_pool_is_resumed_with_the_oracles_counters(schedule, lanes, wlds):
+    """A Phase II pass that runs out of pool is suspended, the pool compacted
+    and the same pass run again before the iteration ends, so the additions,
+    the merged elements and the iteration count stay the oracle's (pms.cu's
+    loop never sees a full pool)."""
+    say("suspended pass", schedule, lanes, wlds)
+    b, ref = _suspend_case()
+    opt = openph_amd.Options()
+    opt.schedule = schedule
+    opt.lanes_per_column = lanes
+    opt.group_lds = wlds
+    opt.pool_elems = b.nnz + b.m + 2048
+    low, ess, st = openph_amd.reduce(b, "pms", opt)
+    assert st.overflows > 0 and st.compactions > 0
+    assert np.array_equal(low, ref.low)
+    assert np.array_equal(ess, ref.ess)
+    assert st.additions == ref.additions
+    assert st.merge_elems == ref.merge_elems
+    assert st.num_iters == ref.num_iters
 
 
 @pytest.mark.parametrize("extra", [0, 1024, 65536])
