@@ -127,8 +127,12 @@ static int compact(Context &c, const RunCfg &cfg, hipStream_t s, int *ncompact, 
     size_t cap = lcap;
     size_t want = cap;
     if (live * 2 > cap) want = std::max<size_t>(cap * 2, live * 2);
+    // the room a compaction leaves free at the least: a small pool grows to
+    // it (within the budget) instead of failing
+    const size_t need = live + live / 16 + 1024;
+    if (want < need) want = need;
     if (want > cfg.pool_max) want = cfg.pool_max;
-    if (live + live / 16 + 1024 > want) return OPENPH_ENOMEM;
+    if (need > want) return OPENPH_ENOMEM;
     OPH_CHECK(grow(&d.pool2, c.cap_pool2, want));
     launch_compact_copy(d, s);
     std::swap(c.cap_pool, c.cap_pool2);

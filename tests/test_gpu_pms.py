@@ -237,6 +237,23 @@ def test_phase2_out_of_pool_is_resumed_with_the_oracles_counters(schedule, lanes
     assert st.num_iters == ref.num_iters
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_tiny_matrix_with_a_pool_below_its_layout(seed):
+    """A pool asked below the initial layout starts full; its compactions
+    grow it to leave room, whatever the matrix's size."""
+    say("tiny pool", seed)
+    rng = np.random.default_rng(100 + seed)
+    b = C.random_simplicial(rng, 11, p_edge=0.6)
+    ref = oracle.pms(b, snapshot=True)
+    opt = openph_amd.Options()
+    opt.pool_elems = b.nnz + b.m + 1
+    low, ess, st = openph_amd.reduce(b, "pms", opt)
+    assert np.array_equal(low, ref.low)
+    assert np.array_equal(ess, ref.ess)
+    assert st.additions == ref.additions
+    assert st.num_iters == ref.num_iters
+
+
 @pytest.mark.parametrize("extra", [0, 1024, 65536])
 def test_pool_budget_too_small_is_an_error_not_a_wrong_answer(extra):
     """With the pool capped near the input size the reduction either finishes
