@@ -13,11 +13,14 @@
 //              with the left rule of row i, so each step of a Phase II
 //              chain finds its operand and its rule with one 16-byte load
 //   work[]     the columns Phase II has to reduce in this iteration
+//   longw[]    the columns handed to the long-column pass; between passes
+//              the compaction's map of published pivots
 // Columns live in a pool of int32 row indices: column j occupies
 // pool[off[j] .. off[j]+len[j]) in ascending order inside a segment of
 // cap[j] >= len[j] entries (at least slot0), so most Phase II results are
 // written back in place; a column that outgrows its segment gets a new one
-// from a bump allocator, and the host compacts the pool when it fills.
+// from a bump allocator, and the host compacts the pool when it fills (inside
+// a Phase II pass too: the pass is suspended, and runs again after it).
 #include "engine.h"
 
 namespace oph {
