@@ -187,13 +187,15 @@ def _i32(x):
     return np.ascontiguousarray(np.asarray(x).reshape(-1), dtype=np.int32)
 
 
-def ph(algstr, r, c, m, col_width=1, low_true=None, device: int = 0, schedule: int = 0):
+def ph(algstr, r, c, m, col_width=1, low_true=None, device: int = 0, schedule: int = 0,
+       options: Options | None = None):
     """The mex gateway's contract (src/matlab/ph/ph.cu:11-92): 1-based COO in,
     eight outputs back. Errors raise OpenPHError instead of printing.
 
     low_true (1-based, 0 = none) feeds the err_* trackers; without it they
     keep the gateway's initial -1. schedule selects how the host drives the
-    pms iterations (Options.schedule; 0 = automatic, as the drop-in does)."""
+    pms iterations (Options.schedule; 0 = automatic, as the drop-in does).
+    options, when given, supplies the other fields (pool, lane shape)."""
     rows = _i32(r)
     cols = _i32(c)
     m = int(m)
@@ -209,6 +211,8 @@ def ph(algstr, r, c, m, col_width=1, low_true=None, device: int = 0, schedule: i
     trk = np.full((5, m + 1), -1.0, np.float32)
     trk[4, :] = 0.0
     opt = Options()
+    if options is not None:
+        ctypes.memmove(ctypes.byref(opt), ctypes.byref(options), ctypes.sizeof(Options))
     opt.device = device
     opt.track = 1
     opt.schedule = int(schedule)

@@ -222,8 +222,9 @@ void launch_track(const Dev &d, const int *low_true0, hipStream_t s);
 // (the first write after the host cleared the ring wins, so instances queued
 // past a halt leave it alone) into its[], and the trackers of the state
 // (d.ltrue) into trk_ring[iter % kIterRing]. Instances queued past a halt
-// recompute the same entry from the same state.
-void launch_iter_end_stamp(const Dev &d, hipStream_t s);
+// recompute the same entry from the same state. `again` overwrites the end
+// clock: an iteration whose suspended Phase II pass has just been finished.
+void launch_iter_end_stamp(const Dev &d, hipStream_t s, int again = 0);
 void launch_track_iter(const Dev &d, hipStream_t s);
 // Zeroes the clock ring before a chunk.
 void launch_clear_stamps(const Dev &d, hipStream_t s);

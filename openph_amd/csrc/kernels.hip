@@ -3052,10 +3052,10 @@ __global__ void __launch_bounds__(1024) k_track_fold(Dev d, int nblocks) {
     }
 }
 
-__global__ void k_iter_end_stamp(Dev d) {
+__global__ void k_iter_end_stamp(Dev d, int again) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     unsigned long long *e = d.its + 2 * (d.ctrl->iter % kIterRing) + 1;
-    if (*e == 0) *e = (unsigned long long)wall_clock64();
+    if (*e == 0 || again) *e = (unsigned long long)wall_clock64();
 }
 
 __global__ void k_zero_u64(unsigned long long *p, int n) {
@@ -3063,7 +3063,9 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
     if (i < n) p[i] = 0;
 }
 
-void launch_iter_end_stamp(const Dev &d, hipStream_t s) { k_iter_end_stamp<<<1, 64, 0, s>>>(d); }
+void launch_iter_end_stamp(const Dev &d, hipStream_t s, int again) {
+    k_iter_end_stamp<<<1, 64, 0, s>>>(d, again);
+}
 
 void launch_track_iter(const Dev &d, hipStream_t s) {
     if (!d.m) return;

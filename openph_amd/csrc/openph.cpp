@@ -166,7 +166,9 @@ static int resume_suspended(Context &c, const RunCfg &cfg, int lanes, int wlds, 
     launch_resume2(d, s);
     launch_phase2(d, lanes, wlds, grid, long_grid, s);
     launch_apply2(d, s);
-    if (d.ltrue) launch_track_iter(d, s);  // the finished iteration's trackers
+    // the finished iteration's end clock and trackers
+    if (d.its) launch_iter_end_stamp(d, s, 1);
+    if (d.ltrue) launch_track_iter(d, s);
     return OPENPH_OK;
 }
 

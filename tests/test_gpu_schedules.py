@@ -236,6 +236,33 @@ def test_trackers_match_oracle(name, schedule):
     assert err_ess[n - 1] == 1
 
 
+@pytest.mark.parametrize("schedule", [1, 2, 3])
+def test_trackers_with_suspended_passes(schedule):
+    """With a pool too small for Phase II, passes are suspended and finished
+    after a compaction; the trackers and iteration times still describe whole
+    iterations, equal to the oracle's."""
+    say("trackers, small pool", schedule)
+    f = np.load(os.path.join(GOLD, "torus100.npz"))
+    b = _gen("torus100")
+    r, c = b.coo1()
+    lt1 = (f["low"] + 1).astype(np.int32)
+    base = openph_amd.Options()
+    base.slot_elems = 1
+    base.pool_elems = b.nnz + b.m + 4096
+    o = openph_amd.ph("pms", r, c, b.m, 8, lt1, schedule=schedule, options=base)
+    low, ess, err_linf, err_lone, err_redu, err_ess, time_track, nit = o
+    n = int(nit[0])
+    ref = oracle.pms(b, snapshot=True, low_true=f["low"].astype(np.int32))
+    assert n == ref.num_iters
+    assert np.array_equal(low, lt1) and np.array_equal(ess, f["ess"])
+    e_linf, e_lone, e_redu, e_ess = _expected_trackers(ref.trk, b.m, lt1, n)
+    assert np.array_equal(err_linf, e_linf)
+    assert np.array_equal(err_lone, e_lone)
+    assert np.array_equal(err_redu, e_redu)
+    assert np.array_equal(err_ess, e_ess)
+    assert time_track[0] == 0 and (time_track[1:n] > 0).all() and (time_track[n:] == 0).all()
+
+
 def test_dropin_void_openph_fills_trackers_like_ph():
     """openph() itself (the C symbol the mex gateway calls) with outputs
     initialised as src/matlab/ph/ph.cu:72-78 does."""
